@@ -1,0 +1,644 @@
+// Clipping kernels: per (tet, candidate site) pair, the tet clipped by the
+// bisectors of the site with its nearest neighbours (the reference's kernel A,
+// voronoi.cu:662-701), the volume and first moment of the result, and the
+// per-site reduction and Lloyd update (voronoi.cu:577-618, 728-749).
+//
+// Two launch forms share the polytope code:
+//   * tet-major ("plain"): lane i of the grid takes pair i = tet*TK + rank, the
+//     reference's pair order; every lane keeps its own planes.
+//   * site-major ("fast"): a wavefront takes the pairs of one site; the site's
+//     bisector planes are built once per wave in LDS and shared by all lanes,
+//     and lanes that finish pick up the site's next pair.
+// Both produce the same per-pair results bit for bit.
+//
+// Polytope state lives in LDS, slot-major ([slot][lane]) so that lanes working
+// on the same slot hit distinct banks.
+#include "vd_clip.hpp"
+#include "vd_geom.hpp"
+
+namespace vd {
+
+namespace {
+
+__device__ __forceinline__ uint32_t pack3(int a, int b, int c) { return (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16); }
+__device__ __forceinline__ int tri_a(uint32_t t) { return (int)(t & 255u); }
+__device__ __forceinline__ int tri_b(uint32_t t) { return (int)((t >> 8) & 255u); }
+__device__ __forceinline__ int tri_c(uint32_t t) { return (int)((t >> 16) & 255u); }
+__device__ __forceinline__ int tri_at(uint32_t t, int k) { return (int)((t >> (8 * k)) & 255u); }
+
+// Storage policies --------------------------------------------------------
+//
+// PlaneStoreLocal: planes [pcap][64] per lane, ids are positions (reference).
+// PlaneStoreShared: tet faces per lane (ids 0..3), bisector v at id 4+v in a
+//   table shared by the wave.  A separate counter nv keeps the reference's
+//   plane count so that vertex_overflow happens exactly where it would.
+
+struct PlaneStoreLocal {
+  float4* pl;  // [pcap][64]
+  int lane;
+  __device__ __forceinline__ P4 get(int i) const { return ld4(pl, i * 64 + lane); }
+  __device__ __forceinline__ void put(int i, const P4& p) { pl[i * 64 + lane] = make_float4(p.x, p.y, p.z, p.w); }
+};
+
+struct PlaneStoreShared {
+  float4* face;        // [4][64] per lane
+  const float4* bis;   // [K] shared
+  int lane;
+  __device__ __forceinline__ P4 get(int i) const { return i < 4 ? ld4(face, i * 64 + lane) : ld4(bis, i - 4); }
+  __device__ __forceinline__ void put(int i, const P4& p) { face[i * 64 + lane] = make_float4(p.x, p.y, p.z, p.w); }
+};
+
+template <class PS>
+struct Cell {
+  PS ps;
+  uint32_t* tr;  // [tcap][64]
+  uint8_t* nx;   // [nxcap][64]
+  int lane;
+  int pcap, tcap;
+  int nv, nt, nr;
+  int first;
+  int status;
+  P4 seed;
+
+  __device__ __forceinline__ uint32_t tri(int t) const { return tr[t * 64 + lane]; }
+  __device__ __forceinline__ void set_tri(int t, uint32_t v) { tr[t * 64 + lane] = v; }
+  __device__ __forceinline__ int nxt(int p) const { return nx[p * 64 + lane]; }
+  __device__ __forceinline__ void set_nxt(int p, int v) { nx[p * 64 + lane] = (uint8_t)v; }
+
+  __device__ __forceinline__ P4 vertex(uint32_t t) const { return vertex3(ps.get(tri_a(t)), ps.get(tri_b(t)), ps.get(tri_c(t))); }
+
+  // Tet start: planes of faces cbd, acd, bad, abc (voronoi.cu:163-201).
+  __device__ void init_tet(const P4& s, const P4 v[4]) {
+    seed = s;
+    status = kSuccess;
+    ps.put(0, tri2plane(v[2], v[1], v[3]));
+    ps.put(1, tri2plane(v[0], v[2], v[3]));
+    ps.put(2, tri2plane(v[1], v[0], v[3]));
+    ps.put(3, tri2plane(v[0], v[1], v[2]));
+    nv = 4;
+    set_tri(0, pack3(1, 3, 2));
+    set_tri(1, pack3(0, 1, 2));
+    set_tri(2, pack3(0, 3, 1));
+    set_tri(3, pack3(0, 2, 3));
+    nt = 4;
+    first = kNoPlane;
+  }
+
+  // Every vertex closer to the seed than half the distance to z
+  // (voronoi.cu:219-232); d2z = |z - seed|^2.
+  __device__ bool security_reached(float d2z) const {
+    float vmax = 0.f;
+    for (int t = 0; t < nt; ++t) {
+      const P4 d = sub3(vertex(tri(t)), seed);
+      vmax = fmaxf(dot3(d, d), vmax);
+    }
+    return d2z > 4.f * vmax;
+  }
+
+  // Boundary loop of the removed triangles tr[nt..nt+nr) (voronoi.cu:391-441):
+  // removed triangles are merged one at a time into a circular list nx[] of
+  // planes; a triangle whose merge would pinch the loop is deferred.
+  __device__ void cavity_boundary() {
+    for (int r = nt; r < nt + nr; ++r) {
+      const uint32_t t = tri(r);
+      set_nxt(tri_a(t), kNoPlane);
+      set_nxt(tri_b(t), kNoPlane);
+      set_nxt(tri_c(t), kNoPlane);
+    }
+    first = kNoPlane;
+    int guard = 0;
+    int cur = nt;
+    while (nr > 0) {
+      if (guard++ > 65535) {
+        status = kInconsistentBoundary;
+        return;
+      }
+      const uint32_t t = tri(cur);
+      const int e0 = tri_a(t), e1 = tri_b(t), e2 = tri_c(t);
+      const int n0 = nxt(e0), n1 = nxt(e1), n2 = nxt(e2);
+      const bool on0 = n0 != kNoPlane, on1 = n1 != kNoPlane, on2 = n2 != kNoPlane;
+      // opp[k]: the loop already runs e[k+1] -> e[k]
+      const bool op0 = n1 == e0, op1 = n2 == e1, op2 = n0 == e2;
+      bool simple = true;
+      if (!op0 && !op1 && on1) simple = false;
+      if (!op1 && !op2 && on2) simple = false;
+      if (!op2 && !op0 && on0) simple = false;
+      if (!op0 && !op1 && !op2) {
+        if (first == kNoPlane) {
+          set_nxt(e0, e1);
+          set_nxt(e1, e2);
+          set_nxt(e2, e0);
+          first = e0;
+        } else {
+          simple = false;
+        }
+      }
+      if (!simple) {
+        ++cur;
+        if (cur == nt + nr) cur = nt;
+        continue;
+      }
+      if (!op0) set_nxt(e0, e1);
+      if (!op1) set_nxt(e1, e2);
+      if (!op2) set_nxt(e2, e0);
+      if (op0 && op1) {
+        if (first == e1) first = nxt(e1);
+        set_nxt(e1, kNoPlane);
+      }
+      if (op1 && op2) {
+        if (first == e2) first = nxt(e2);
+        set_nxt(e2, kNoPlane);
+      }
+      if (op2 && op0) {
+        if (first == e0) first = nxt(e0);
+        set_nxt(e0, kNoPlane);
+      }
+      const int last = nt + nr - 1;
+      const uint32_t tl = tri(last);
+      set_tri(last, t);
+      set_tri(cur, tl);
+      cur = nt;
+      --nr;
+    }
+  }
+
+  // Clip by plane eqn stored under id `pid` (voronoi.cu:443-486).
+  __device__ void clip(int pid, const P4& eqn) {
+    if (nv >= pcap) {
+      status = kVertexOverflow;
+      return;
+    }
+    ++nv;
+    nr = 0;
+    int i = 0;
+    while (i < nt) {
+      const uint32_t t = tri(i);
+      const Minors m = vertex_minors(ps.get(tri_a(t)), ps.get(tri_b(t)), ps.get(tri_c(t)));
+      if (conflict_det(m, eqn) > 0.f) {
+        --nt;
+        set_tri(i, tri(nt));
+        set_tri(nt, t);
+        ++nr;
+      } else {
+        ++i;
+      }
+    }
+    if (nr == 0) {
+      --nv;
+      return;
+    }
+    if (nt == 0) {
+      status = kNoIntersection;
+      return;
+    }
+    cavity_boundary();
+    if (status != kSuccess) return;
+    if (first == kNoPlane) return;
+    int c = first;
+    int guard = 0;
+    do {
+      if (c == kNoPlane || guard++ > pcap) {
+        status = kInconsistentBoundary;
+        return;
+      }
+      if (nt + 1 >= tcap) {
+        status = kTriangleOverflow;
+        return;
+      }
+      const int n = nxt(c);
+      if (n == kNoPlane) {
+        status = kInconsistentBoundary;
+        return;
+      }
+      set_tri(nt, pack3(pid, c, n));
+      ++nt;
+      c = n;
+    } while (c != first);
+  }
+
+  // Volume and first moment: the reference's six-tet-per-vertex decomposition
+  // around the seed (voronoi.cu:532-537, 577-618).
+  template <int ACC>
+  __device__ void moments_ref(PairSums<ACC>& S) const {
+    const P4 C = seed;
+    for (int t = 0; t < nt; ++t) {
+      const uint32_t tt = tri(t);
+      const P4 q0 = ps.get(tri_a(tt)), q1 = ps.get(tri_b(tt)), q2 = ps.get(tri_c(tt));
+      const P4 X = vertex3(q0, q1, q2);
+      P4 P[6];
+      P[0] = project_ref(C, q0);
+      P[2] = project_ref(C, q1);
+      P[4] = project_ref(C, q2);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        P4 n = cross3(sub3(P[2 * k], C), sub3(P[(2 * k + 2) % 6], C));
+        n.w = -dot3(C, n);
+        P[2 * k + 1] = project_ref(X, n);
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        float v;
+        P4 b;
+        tet_vol_bary(P[k], P[(k + 1) % 6], C, X, v, b);
+        S.add(v, b);
+      }
+    }
+  }
+
+  // Same ring around the mean of the vertices, with edge feet taken on the
+  // edge line (the library's well-conditioned formulation).
+  template <int ACC>
+  __device__ void moments_robust(PairSums<ACC>& S) const {
+    P4 O = {0.f, 0.f, 0.f, 1.f};
+    for (int t = 0; t < nt; ++t) {
+      const P4 X = vertex(tri(t));
+      O.x += X.x;
+      O.y += X.y;
+      O.z += X.z;
+    }
+    const float inv = 1.f / (float)(nt > 0 ? nt : 1);
+    O.x *= inv;
+    O.y *= inv;
+    O.z *= inv;
+    for (int t = 0; t < nt; ++t) {
+      const uint32_t tt = tri(t);
+      const P4 q[3] = {ps.get(tri_a(tt)), ps.get(tri_b(tt)), ps.get(tri_c(tt))};
+      const P4 X = vertex3(q[0], q[1], q[2]);
+      P4 P[6];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) P[2 * k] = foot_on_plane(O, q[k]);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) P[2 * k + 1] = foot_on_line(O, X, cross3(q[k], q[(k + 1) % 3]));
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        float v;
+        P4 b;
+        tet_vol_bary(P[k], P[(k + 1) % 6], O, X, v, b);
+        S.add(v, b);
+      }
+    }
+  }
+};
+
+// Result of one pair, written to the per-site accumulators.
+template <int ACC>
+__device__ __forceinline__ void accumulate(const ClipArgs& a, int site, const PairSums<ACC>& S);
+
+template <>
+__device__ __forceinline__ void accumulate<0>(const ClipArgs& a, int site, const PairSums<0>& S) {
+  float* p = a.acc_f32 + 4 * (size_t)site;
+  unsafeAtomicAdd(p + 0, S.v);
+  unsafeAtomicAdd(p + 1, S.x);
+  unsafeAtomicAdd(p + 2, S.y);
+  unsafeAtomicAdd(p + 3, S.z);
+}
+template <>
+__device__ __forceinline__ void accumulate<1>(const ClipArgs& a, int site, const PairSums<1>& S) {
+  unsigned long long* p = a.acc_fix + 4 * (size_t)site;
+  atomicAdd(p + 0, (unsigned long long)__double2ll_rn(S.v * kFixScale));
+  atomicAdd(p + 1, (unsigned long long)__double2ll_rn(S.x * kFixScale));
+  atomicAdd(p + 2, (unsigned long long)__double2ll_rn(S.y * kFixScale));
+  atomicAdd(p + 3, (unsigned long long)__double2ll_rn(S.z * kFixScale));
+}
+
+__device__ __forceinline__ void count_status(const ClipArgs& a, int status, bool live) {
+  if (!a.status_hist) return;
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int s = 0; s < 7; ++s) {
+    const unsigned long long m = __ballot(live && status == s);
+    if (lane == 0 && m) atomicAdd(a.status_hist + s, (unsigned long long)__popcll(m));
+  }
+}
+
+__device__ __forceinline__ P4 site_at(const ClipArgs& a, int i) { return ld4(a.sites, i); }
+
+// Clips one pair whose cell has been set up; returns through S and status.
+// kernel A's loop (voronoi.cu:679-700) with the bisector of the v-th neighbour
+// under plane id pid(v).
+template <int MF, int ACC, class PS, class PlaneOf>
+__device__ __forceinline__ bool run_pair(const ClipArgs& a, Cell<PS>& c, const int32_t* nbr, PlaneOf plane_of,
+                                         PairSums<ACC>& S) {
+  const int K = a.K;
+  bool stopped = false;
+  for (int v = 0; v < K; ++v) {
+    const int z = nbr[v];
+    if ((unsigned)z >= (unsigned)a.n_sites) {  // defensive: never read outside the site array
+      c.status = kInconsistentBoundary;
+      return false;
+    }
+    int pid;
+    P4 eq;
+    float d2z;
+    plane_of(v, z, pid, eq, d2z);
+    c.clip(pid, eq);
+    if (c.security_reached(d2z)) {
+      stopped = true;
+      break;
+    }
+    if (c.status != kSuccess) return false;
+  }
+  if (!stopped || K == 0) {
+    // the last neighbour's test, as the reference makes it after the loop
+    bool ok = false;
+    if (K > 0 && (unsigned)nbr[K - 1] < (unsigned)a.n_sites) {
+      int pid;
+      P4 eq;
+      float d2z;
+      plane_of(K - 1, nbr[K - 1], pid, eq, d2z);
+      ok = c.security_reached(d2z);
+    }
+    if (!ok) c.status = kSecurityRadiusNotReached;
+  }
+  if (c.status == kNoIntersection) return false;
+  if (MF == 0)
+    c.template moments_ref<ACC>(S);
+  else
+    c.template moments_robust<ACC>(S);
+  if (fabs((double)S.v) < 0.1) {
+    c.status = kNoIntersection;
+    return false;
+  }
+  return true;
+}
+
+// ------------------------------------------------------------ tet-major ----
+
+template <int MF, int ACC>
+__global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const int lane = threadIdx.x;
+  Cell<PlaneStoreLocal> c;
+  c.ps.pl = (float4*)smem;
+  c.ps.lane = lane;
+  c.tr = (uint32_t*)(c.ps.pl + (size_t)a.pcap * 64);
+  c.nx = (uint8_t*)(c.tr + (size_t)a.tcap * 64);
+  c.lane = lane;
+  c.pcap = a.pcap;
+  c.tcap = a.tcap;
+  const long long pair = (long long)blockIdx.x * 64 + lane;
+  const bool live = pair < a.n_pairs;
+  int status = kNoIntersection;
+  PairSums<ACC> S;
+  bool acc = false;
+  int seed_id = 0;
+  if (live && (unsigned)a.tet_knn[pair] >= (unsigned)a.n_sites) {
+    status = kInconsistentBoundary;  // defensive: candidate id out of range
+    if (a.pair_status) a.pair_status[pair] = (uint8_t)status;
+  } else if (live) {
+    const int t = (int)(pair / a.TK);
+    seed_id = a.tet_knn[pair];
+    const int4 tv = a.tets[t];
+    const P4 v[4] = {ld4(a.verts, tv.x), ld4(a.verts, tv.y), ld4(a.verts, tv.z), ld4(a.verts, tv.w)};
+    const P4 s = site_at(a, seed_id);
+    c.init_tet(s, v);
+    const int32_t* nbr = a.site_knn + (size_t)seed_id * a.K;
+    auto plane_of = [&](int v, int z, int& pid, P4& eq, float& d2z) {
+      const P4 zp = site_at(a, z);
+      eq = bisector(s, zp);
+      const P4 d = sub3(zp, s);
+      d2z = dot3(d, d);
+      pid = c.nv;  // next free position
+      if (pid < a.pcap) c.ps.put(pid, eq);
+    };
+    acc = run_pair<MF, ACC>(a, c, nbr, plane_of, S);
+    status = c.status;
+    if (acc) accumulate<ACC>(a, seed_id, S);
+    if (a.pair_status) a.pair_status[pair] = (uint8_t)status;
+    if (a.pair_vals) {
+      double* pv = a.pair_vals + 4 * pair;
+      pv[0] = acc ? (double)S.v : 0.0;
+      pv[1] = acc ? (double)S.x : 0.0;
+      pv[2] = acc ? (double)S.y : 0.0;
+      pv[3] = acc ? (double)S.z : 0.0;
+    }
+  }
+  count_status(a, status, live);
+}
+
+// ----------------------------------------------------------- site-major ----
+//
+// Pairs grouped by seed: site s owns pair_ids[pair_start[s] .. pair_start[s+1]).
+// One wavefront per site (grid-strided over sites); lanes take the site's pairs
+// 64 at a time.
+
+template <int MF, int ACC>
+__global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const int lane = threadIdx.x;
+  const int K = a.K;
+  float4* bis = (float4*)smem;                          // [K]
+  float* bd2 = (float*)(bis + K);                       // [K]
+  float4* face = (float4*)(bd2 + ((K + 3) & ~3));       // [4][64]
+  uint32_t* tr = (uint32_t*)(face + 4 * 64);            // [tcap][64]
+  uint8_t* nx = (uint8_t*)(tr + (size_t)a.tcap * 64);   // [4+K][64]
+  Cell<PlaneStoreShared> c;
+  c.ps.face = face;
+  c.ps.bis = bis;
+  c.ps.lane = lane;
+  c.tr = tr;
+  c.nx = nx;
+  c.lane = lane;
+  c.pcap = a.pcap;
+  c.tcap = a.tcap;
+  for (int s = blockIdx.x; s < a.n_sites; s += gridDim.x) {
+    const int pb = a.pair_start[s], pe = a.pair_start[s + 1];
+    if (pb == pe) continue;
+    const P4 sp = site_at(a, s);
+    const int32_t* nbr = a.site_knn + (size_t)s * K;
+    __syncthreads();
+    for (int v = lane; v < K; v += 64) {
+      const int z = nbr[v];
+      if ((unsigned)z >= (unsigned)a.n_sites) continue;  // run_pair rejects such a neighbour
+      const P4 zp = site_at(a, z);
+      const P4 eq = bisector(sp, zp);
+      bis[v] = make_float4(eq.x, eq.y, eq.z, eq.w);
+      const P4 d = sub3(zp, sp);
+      bd2[v] = dot3(d, d);
+    }
+    __syncthreads();
+    for (int base = pb; base < pe; base += 64) {
+      const int i = base + lane;
+      const bool live = i < pe;
+      int status = kNoIntersection;
+      if (live) {
+        const long long pair = a.pair_ids[i];
+        const int t = (int)(pair / a.TK);
+        const int4 tv = a.tets[t];
+        const P4 v[4] = {ld4(a.verts, tv.x), ld4(a.verts, tv.y), ld4(a.verts, tv.z), ld4(a.verts, tv.w)};
+        c.init_tet(sp, v);
+        auto plane_of = [&](int vv, int z, int& pid, P4& eq, float& d2z) {
+          (void)z;
+          pid = 4 + vv;
+          eq = ld4(bis, vv);
+          d2z = bd2[vv];
+        };
+        PairSums<ACC> S;
+        const bool acc = run_pair<MF, ACC>(a, c, nbr, plane_of, S);
+        status = c.status;
+        if (acc) accumulate<ACC>(a, s, S);
+        if (a.pair_status) a.pair_status[pair] = (uint8_t)status;
+        if (a.pair_vals) {
+          double* pv = a.pair_vals + 4 * pair;
+          pv[0] = acc ? (double)S.v : 0.0;
+          pv[1] = acc ? (double)S.x : 0.0;
+          pv[2] = acc ? (double)S.y : 0.0;
+          pv[3] = acc ? (double)S.z : 0.0;
+        }
+      }
+      count_status(a, status, live);
+    }
+  }
+}
+
+// Pairs grouped by seed: counting sort of the candidate table by site.
+__global__ void k_pairs_count(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_pairs;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int s = tet_knn[i];
+    if ((unsigned)s < (unsigned)n_sites) atomicAdd(count + s, 1);
+  }
+}
+__global__ void k_pairs_scatter(const int32_t* tet_knn, long long n_pairs, int n_sites, const int* start,
+                                int* cursor, int* pair_ids) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_pairs;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int s = tet_knn[i];
+    if ((unsigned)s < (unsigned)n_sites) pair_ids[start[s] + atomicAdd(cursor + s, 1)] = (int)i;
+  }
+}
+// Sorts each site's pair list (one thread per site; lists are short).
+__global__ void k_pairs_sort(int n_sites, const int* start, int* pair_ids) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_sites) return;
+  const int b = start[s], e = start[s + 1];
+  for (int i = b + 1; i < e; ++i) {
+    const int v = pair_ids[i];
+    int j = i - 1;
+    while (j >= b && pair_ids[j] > v) {
+      pair_ids[j + 1] = pair_ids[j];
+      --j;
+    }
+    pair_ids[j + 1] = v;
+  }
+}
+
+// Exclusive scan over n counts, single workgroup (n up to a few million).
+__global__ __launch_bounds__(1024) void k_scan1(const int* count, int n, int* start, int* cursor) {
+  __shared__ int part[1024];
+  const int per = (n + 1023) / 1024;
+  const int b = threadIdx.x * per, e = min(n, b + per);
+  int s = 0;
+  for (int i = b; i < e; ++i) s += count[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+  for (int i = b; i < e; ++i) {
+    start[i] = run;
+    run += count[i];
+    cursor[i] = 0;
+  }
+  if (threadIdx.x == 1023) start[n] = part[1023];
+}
+
+// Lloyd update (voronoi.cu:728-749): site = moment / volume where volume != 0.
+__global__ void k_update_f32(float4* sites, int n, const float* acc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float v = acc[4 * i];
+  if (v != 0.f) {
+    float4 p = sites[i];
+    p.x = acc[4 * i + 1] / v;
+    p.y = acc[4 * i + 2] / v;
+    p.z = acc[4 * i + 3] / v;
+    sites[i] = p;
+  }
+}
+__global__ void k_update_fix(float4* sites, int n, const unsigned long long* acc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v = (double)(long long)acc[4 * i] / kFixScale;
+  if (v != 0.0) {
+    float4 p = sites[i];
+    p.x = (float)(((double)(long long)acc[4 * i + 1] / kFixScale) / v);
+    p.y = (float)(((double)(long long)acc[4 * i + 2] / kFixScale) / v);
+    p.z = (float)(((double)(long long)acc[4 * i + 3] / kFixScale) / v);
+    sites[i] = p;
+  }
+}
+// Per-site sums as FP64 (vol, mx, my, mz), for readback.
+__global__ void k_moments_out(int n, int acc_mode, const float* acc_f32, const unsigned long long* acc_fix,
+                              double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 4 * n) return;
+  out[i] = acc_mode == 0 ? (double)acc_f32[i] : (double)(long long)acc_fix[i] / kFixScale;
+}
+
+}  // namespace
+
+size_t clip_plain_lds(int pcap, int tcap) { return (size_t)64 * (16 * pcap + 4 * tcap + pcap); }
+size_t clip_site_lds(int K, int tcap) {
+  return (size_t)16 * K + 4 * (size_t)((K + 3) & ~3) + 16 * 4 * 64 + (size_t)4 * tcap * 64 + (size_t)(4 + K) * 64;
+}
+
+template <class Kern>
+static hipError_t launch_lds(Kern k, dim3 grid, dim3 block, size_t lds, hipStream_t st, const ClipArgs& a) {
+  hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, grid, block, lds, st, a);
+  return hipGetLastError();
+}
+
+hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, int n_blocks, hipStream_t st) {
+  if (a.n_pairs <= 0) return hipSuccess;
+  if (form == 0) {
+    const size_t lds = clip_plain_lds(a.pcap, a.tcap);
+    const dim3 grid((unsigned)((a.n_pairs + 63) / 64)), block(64);
+    if (moments == 0 && acc_mode == 0) return launch_lds(k_clip_plain<0, 0>, grid, block, lds, st, a);
+    if (moments == 0 && acc_mode == 1) return launch_lds(k_clip_plain<0, 1>, grid, block, lds, st, a);
+    if (moments == 1 && acc_mode == 0) return launch_lds(k_clip_plain<1, 0>, grid, block, lds, st, a);
+    return launch_lds(k_clip_plain<1, 1>, grid, block, lds, st, a);
+  }
+  const size_t lds = clip_site_lds(a.K, a.tcap);
+  const dim3 grid((unsigned)n_blocks), block(64);
+  if (moments == 0 && acc_mode == 0) return launch_lds(k_clip_site<0, 0>, grid, block, lds, st, a);
+  if (moments == 0 && acc_mode == 1) return launch_lds(k_clip_site<0, 1>, grid, block, lds, st, a);
+  if (moments == 1 && acc_mode == 0) return launch_lds(k_clip_site<1, 0>, grid, block, lds, st, a);
+  return launch_lds(k_clip_site<1, 1>, grid, block, lds, st, a);
+}
+
+hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count, int* start,
+                         int* cursor, int* pair_ids, hipStream_t st) {
+  hipLaunchKernelGGL(k_pairs_count, dim3(2048), dim3(256), 0, st, tet_knn, n_pairs, n_sites, count);
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, st, count, n_sites, start, cursor);
+  hipLaunchKernelGGL(k_pairs_scatter, dim3(2048), dim3(256), 0, st, tet_knn, n_pairs, n_sites, start, cursor,
+                     pair_ids);
+  hipLaunchKernelGGL(k_pairs_sort, dim3((n_sites + 255) / 256), dim3(256), 0, st, n_sites, start, pair_ids);
+  return hipGetLastError();
+}
+
+hipError_t lloyd_update(float4* sites, int n, int acc_mode, const float* acc_f32, const unsigned long long* acc_fix,
+                        hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  if (acc_mode == 0)
+    hipLaunchKernelGGL(k_update_f32, dim3((n + 255) / 256), dim3(256), 0, st, sites, n, acc_f32);
+  else
+    hipLaunchKernelGGL(k_update_fix, dim3((n + 255) / 256), dim3(256), 0, st, sites, n, acc_fix);
+  return hipGetLastError();
+}
+
+hipError_t moments_out(int n, int acc_mode, const float* acc_f32, const unsigned long long* acc_fix, double* out,
+                       hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_moments_out, dim3((4 * n + 255) / 256), dim3(256), 0, st, n, acc_mode, acc_f32, acc_fix,
+                     out);
+  return hipGetLastError();
+}
+
+}  // namespace vd

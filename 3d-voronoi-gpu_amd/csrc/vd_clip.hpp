@@ -1,0 +1,45 @@
+// Internal interface of the clipping kernels (vd_clip.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vd {
+
+struct ClipArgs {
+  const float4* sites;       // n_sites (x, y, z, _), input order
+  const int32_t* site_knn;   // n_sites x K, nearest first, self excluded
+  int K;
+  const float4* verts;       // (x, y, z, _)
+  const int4* tets;          // 4 vertex ids, positively oriented
+  const int32_t* tet_knn;    // n_tets x TK candidate sites
+  int TK;
+  long long n_pairs;         // n_tets * TK
+  int n_sites;
+  const int* pair_start;     // site-major form: n_sites + 1
+  const int* pair_ids;       // site-major form: n_pairs pair indices grouped by site
+  int pcap, tcap;            // polytope caps (planes, vertices)
+  float* acc_f32;            // n_sites x 4 (vol, mx, my, mz), FP32 accumulation
+  unsigned long long* acc_fix;  // n_sites x 4, 2^-20 fixed point of FP64 pair sums
+  uint8_t* pair_status;      // optional, n_pairs
+  double* pair_vals;         // optional, n_pairs x 4
+  unsigned long long* status_hist;  // optional, 7 counters
+};
+
+size_t clip_plain_lds(int pcap, int tcap);
+size_t clip_site_lds(int K, int tcap);
+
+// form: 0 tet-major, 1 site-major; moments: 0 reference, 1 robust;
+// acc_mode: 0 FP32 atomics, 1 FP64 pair sums + fixed-point site sums.
+hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, int n_blocks, hipStream_t st);
+
+// Groups the pairs of an n_tets x TK candidate table by site (deterministic).
+hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count, int* start,
+                         int* cursor, int* pair_ids, hipStream_t st);
+
+hipError_t lloyd_update(float4* sites, int n, int acc_mode, const float* acc_f32, const unsigned long long* acc_fix,
+                        hipStream_t st);
+hipError_t moments_out(int n, int acc_mode, const float* acc_f32, const unsigned long long* acc_fix, double* out,
+                       hipStream_t st);
+
+}  // namespace vd

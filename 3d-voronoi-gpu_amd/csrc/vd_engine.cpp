@@ -1,0 +1,571 @@
+// Host engine of libvoronoi_hip.so: contexts, device buffers, the Lloyd
+// iteration and the C ABI declared in include/vd_api.h.
+//
+// One context = one mesh + one site set resident in HBM.  An iteration is
+//   grid build -> site kNN -> candidate kNN -> (pairs by site) -> clip -> update
+// all enqueued on the context's stream with no host round trip; the host only
+// reads results when asked.  Layout in HBM:
+//   verts float4[n_verts], tets int4[n_tets], sites float4[n_sites]
+//   site_knn int32[n_sites][K], tet_knn int32[n_tets][TK]
+//   grid: sorted float4[n_sites] + ids + cell starts
+//   accumulators: float[n_sites][4] or uint64[n_sites][4]
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "vd_api.h"
+#include "vd_clip.hpp"
+#include "vd_knn.hpp"
+
+namespace {
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= cap && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc((void**)&p, bytes);
+    if (e == hipSuccess) cap = std::max<size_t>(n, 1);
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+int tet_k_heuristic(int n_site, int n_tet, int preferred) {
+  // voronoi.cu:936
+  if (preferred) return preferred;
+  if (n_tet <= 0) return 20;
+  const int h = 10 * (int)std::ceil(8.0 * std::log10(7.0 * n_site / n_tet + 1.0));
+  return std::max(h, 20);
+}
+
+}  // namespace
+
+struct vd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  vd_config cfg{};
+  std::string err;
+
+  int n_verts = 0, n_tets = 0, n_sites = 0;
+  int K = 0, TK = 0;
+  bool have_knn = false, have_clip = false;
+  float mesh_lo[3] = {0, 0, 0}, mesh_hi[3] = {0, 0, 0};
+
+  DevBuf<float4> verts, sites;
+  DevBuf<int4> tets;
+  DevBuf<int4> offsets;
+  int n_offsets = 0;
+  // grid
+  DevBuf<vd::GridShape> shape;
+  DevBuf<int> g_count, g_start, g_cursor, g_cell_of, g_sorted_ids;
+  DevBuf<float4> g_sorted_pts;
+  int max_cells = 0;
+  // kNN results
+  DevBuf<int32_t> site_knn, tet_knn;
+  // site-major pairs
+  DevBuf<int> p_count, p_start, p_cursor, p_ids;
+  // accumulators
+  DevBuf<float> acc_f32;
+  DevBuf<unsigned long long> acc_fix;
+  DevBuf<double> moments;
+  DevBuf<uint8_t> pair_status;
+  DevBuf<double> pair_vals;
+  DevBuf<unsigned long long> hist;
+
+  // HIP events of the phases of the last kRing iterations: slot = iteration % kRing;
+  // [0] start, [1] after kNN, [2] after pair grouping, [3] after clip, [4] after update
+  static constexpr int kRing = 256;
+  hipEvent_t ev[5][kRing] = {};
+  int it_slot = 0;        // slot of the iteration being enqueued
+  long long it_done = 0;  // iterations completed since the last timing reset
+  bool ev_knn = false, ev_pairs = false, ev_clip = false, ev_update = false;
+  hipEvent_t& E(int phase) { return ev[phase][it_slot]; }
+
+  int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+  int hip_fail(hipError_t e, const char* what) {
+    return fail(e == hipErrorOutOfMemory ? VD_ERR_NOMEM : VD_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+  }
+};
+
+#define VD_HIP(call, what)                          \
+  do {                                              \
+    hipError_t e_ = (call);                         \
+    if (e_ != hipSuccess) return ctx->hip_fail(e_, what); \
+  } while (0)
+
+namespace {
+
+int check_config(vd_ctx* ctx, const vd_config& c) {
+  if (c.site_k < 1 || c.site_k > 256) return ctx->fail(VD_ERR_ARG, "site_k must be in [1,256], got %d", c.site_k);
+  if (c.tet_k < 0 || c.tet_k > 256) return ctx->fail(VD_ERR_ARG, "tet_k must be in [0,256], got %d", c.tet_k);
+  if (c.pcap < 8 || c.pcap > 254) return ctx->fail(VD_ERR_ARG, "pcap must be in [8,254], got %d", c.pcap);
+  if (c.tcap < 8 || c.tcap > 1024) return ctx->fail(VD_ERR_ARG, "tcap must be in [8,1024], got %d", c.tcap);
+  if (c.accum != VD_ACCUM_F32 && c.accum != VD_ACCUM_F64) return ctx->fail(VD_ERR_ARG, "bad accum %d", c.accum);
+  if (c.moments != VD_MOMENTS_REFERENCE && c.moments != VD_MOMENTS_ROBUST)
+    return ctx->fail(VD_ERR_ARG, "bad moments %d", c.moments);
+  if (c.form != VD_FORM_TET_MAJOR && c.form != VD_FORM_SITE_MAJOR) return ctx->fail(VD_ERR_ARG, "bad form %d", c.form);
+  if (!(c.grid_density > 0.05f && c.grid_density < 1000.f))
+    return ctx->fail(VD_ERR_ARG, "grid_density out of range");
+  if (c.form == VD_FORM_TET_MAJOR && vd::clip_plain_lds(c.pcap, c.tcap) > 160 * 1024)
+    return ctx->fail(VD_ERR_ARG, "pcap/tcap too large for the tet-major form's LDS");
+  if (c.form == VD_FORM_SITE_MAJOR && vd::clip_site_lds(c.site_k, c.tcap) > 160 * 1024)
+    return ctx->fail(VD_ERR_ARG, "site_k/tcap too large for the site-major form's LDS");
+  if (c.form == VD_FORM_SITE_MAJOR && c.site_k + 4 > 254)
+    return ctx->fail(VD_ERR_ARG, "site-major form needs site_k <= 250");
+  return VD_OK;
+}
+
+int ensure_grid(vd_ctx* ctx, int n) {
+  const int mc = vd::grid_max_cells(n, ctx->cfg.grid_density);
+  VD_HIP(ctx->shape.ensure(1), "alloc grid shape");
+  VD_HIP(ctx->g_count.ensure(mc), "alloc grid");
+  VD_HIP(ctx->g_start.ensure((size_t)mc + 1), "alloc grid");
+  VD_HIP(ctx->g_cursor.ensure(mc), "alloc grid");
+  VD_HIP(ctx->g_cell_of.ensure(n), "alloc grid");
+  VD_HIP(ctx->g_sorted_ids.ensure(n), "alloc grid");
+  VD_HIP(ctx->g_sorted_pts.ensure(n), "alloc grid");
+  ctx->max_cells = mc;
+  return VD_OK;
+}
+
+vd::GridBufs grid_bufs(vd_ctx* ctx) {
+  vd::GridBufs gb;
+  gb.shape = ctx->shape.p;
+  gb.max_cells = ctx->max_cells;
+  gb.count = ctx->g_count.p;
+  gb.start = ctx->g_start.p;
+  gb.cursor = ctx->g_cursor.p;
+  gb.cell_of = ctx->g_cell_of.p;
+  gb.sorted_ids = ctx->g_sorted_ids.p;
+  gb.sorted_pts = ctx->g_sorted_pts.p;
+  return gb;
+}
+
+int upload_xyz(vd_ctx* ctx, DevBuf<float4>& dst, const float* xyz, int n) {
+  std::vector<float4> h((size_t)std::max(n, 1));
+  for (int i = 0; i < n; ++i) h[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.f);
+  VD_HIP(dst.ensure(std::max(n, 1)), "alloc points");
+  if (n > 0)
+    VD_HIP(hipMemcpyAsync(dst.p, h.data(), sizeof(float4) * n, hipMemcpyHostToDevice, ctx->stream), "upload points");
+  VD_HIP(hipStreamSynchronize(ctx->stream), "upload points");
+  return VD_OK;
+}
+
+int ensure_offsets(vd_ctx* ctx) {
+  if (ctx->n_offsets) return VD_OK;
+  const std::vector<int4> t = vd::knn_offset_table();
+  VD_HIP(ctx->offsets.ensure(t.size()), "alloc offsets");
+  VD_HIP(hipMemcpy(ctx->offsets.p, t.data(), sizeof(int4) * t.size(), hipMemcpyHostToDevice), "upload offsets");
+  ctx->n_offsets = (int)t.size();
+  return VD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void vd_config_default(vd_config* c) {
+  if (!c) return;
+  c->site_k = 90;
+  c->tet_k = 0;
+  c->pcap = 64;
+  c->tcap = 96;
+  c->accum = VD_ACCUM_F32;
+  c->moments = VD_MOMENTS_REFERENCE;
+  c->form = VD_FORM_SITE_MAJOR;
+  c->grid_density = 3.1f;
+  c->record_pairs = 0;
+  c->clip_blocks = 0;
+}
+
+int vd_ctx_create(vd_ctx** out, int device, const vd_config* cfg) {
+  if (!out) return VD_ERR_ARG;
+  *out = nullptr;
+  vd_ctx* ctx = new vd_ctx();
+  ctx->device = device;
+  if (cfg)
+    ctx->cfg = *cfg;
+  else
+    vd_config_default(&ctx->cfg);
+  int rc = check_config(ctx, ctx->cfg);
+  if (rc != VD_OK) {
+    fprintf(stderr, "vd_ctx_create: %s\n", ctx->err.c_str());
+    delete ctx;
+    return rc;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    fprintf(stderr, "vd_ctx_create: no HIP device %d (found %d)\n", device, ndev);
+    delete ctx;
+    return VD_ERR_HIP;
+  }
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    fprintf(stderr, "vd_ctx_create: cannot create a stream on device %d\n", device);
+    delete ctx;
+    return VD_ERR_HIP;
+  }
+  for (auto& row : ctx->ev)
+    for (auto& e : row) (void)hipEventCreate(&e);
+  if (ensure_offsets(ctx) != VD_OK || ctx->hist.ensure(8) != hipSuccess) {
+    fprintf(stderr, "vd_ctx_create: %s\n", ctx->err.c_str());
+    vd_ctx_destroy(ctx);
+    return VD_ERR_HIP;
+  }
+  *out = ctx;
+  return VD_OK;
+}
+
+void vd_ctx_destroy(vd_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->verts.release(); ctx->sites.release(); ctx->tets.release(); ctx->offsets.release();
+  ctx->shape.release(); ctx->g_count.release(); ctx->g_start.release(); ctx->g_cursor.release();
+  ctx->g_cell_of.release(); ctx->g_sorted_ids.release(); ctx->g_sorted_pts.release();
+  ctx->site_knn.release(); ctx->tet_knn.release();
+  ctx->p_count.release(); ctx->p_start.release(); ctx->p_cursor.release(); ctx->p_ids.release();
+  ctx->acc_f32.release(); ctx->acc_fix.release(); ctx->moments.release();
+  ctx->pair_status.release(); ctx->pair_vals.release(); ctx->hist.release();
+  for (auto& row : ctx->ev)
+    for (auto& e : row)
+      if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* vd_ctx_error(const vd_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int vd_ctx_set_config(vd_ctx* ctx, const vd_config* cfg) {
+  if (!ctx || !cfg) return VD_ERR_ARG;
+  int rc = check_config(ctx, *cfg);
+  if (rc != VD_OK) return rc;
+  const bool grid_changed = cfg->grid_density != ctx->cfg.grid_density;
+  ctx->cfg = *cfg;
+  if (grid_changed && ctx->n_sites > 0) {
+    rc = ensure_grid(ctx, ctx->n_sites);
+    if (rc != VD_OK) return rc;
+  }
+  ctx->have_knn = false;
+  ctx->have_clip = false;
+  return VD_OK;
+}
+
+int vd_set_mesh(vd_ctx* ctx, const float* verts_xyz, int n_verts, const int32_t* tets, int n_tets) {
+  if (!ctx || n_verts < 0 || n_tets < 0 || (n_verts && !verts_xyz) || (n_tets && !tets))
+    return ctx ? ctx->fail(VD_ERR_ARG, "vd_set_mesh: bad arguments") : VD_ERR_ARG;
+  VD_HIP(hipSetDevice(ctx->device), "set device");
+  for (long long i = 0; i < 4LL * n_tets; ++i)
+    if (tets[i] < 0 || tets[i] >= n_verts)
+      return ctx->fail(VD_ERR_PRECONDITION, "tet %lld refers to vertex %d of %d", i / 4, tets[i], n_verts);
+  for (int a = 0; a < 3; ++a) {
+    ctx->mesh_lo[a] = n_verts ? verts_xyz[a] : 0.f;
+    ctx->mesh_hi[a] = n_verts ? verts_xyz[a] : 0.f;
+  }
+  for (int i = 0; i < n_verts; ++i)
+    for (int a = 0; a < 3; ++a) {
+      ctx->mesh_lo[a] = std::min(ctx->mesh_lo[a], verts_xyz[3 * i + a]);
+      ctx->mesh_hi[a] = std::max(ctx->mesh_hi[a], verts_xyz[3 * i + a]);
+    }
+  int rc = upload_xyz(ctx, ctx->verts, verts_xyz, n_verts);
+  if (rc != VD_OK) return rc;
+  VD_HIP(ctx->tets.ensure(std::max(n_tets, 1)), "alloc tets");
+  if (n_tets > 0)
+    VD_HIP(hipMemcpy(ctx->tets.p, tets, sizeof(int4) * (size_t)n_tets, hipMemcpyHostToDevice), "upload tets");
+  ctx->n_verts = n_verts;
+  ctx->n_tets = n_tets;
+  ctx->have_knn = ctx->have_clip = false;
+  return VD_OK;
+}
+
+int vd_set_sites(vd_ctx* ctx, const float* sites_xyz, int n_sites) {
+  if (!ctx || n_sites < 0 || (n_sites && !sites_xyz)) return ctx ? ctx->fail(VD_ERR_ARG, "vd_set_sites: bad arguments") : VD_ERR_ARG;
+  VD_HIP(hipSetDevice(ctx->device), "set device");
+  int rc = upload_xyz(ctx, ctx->sites, sites_xyz, n_sites);
+  if (rc != VD_OK) return rc;
+  rc = ensure_grid(ctx, std::max(n_sites, 1));
+  if (rc != VD_OK) return rc;
+  ctx->n_sites = n_sites;
+  ctx->have_knn = ctx->have_clip = false;
+  return VD_OK;
+}
+
+int vd_get_sites(vd_ctx* ctx, float* out) {
+  if (!ctx || (!out && ctx->n_sites)) return VD_ERR_ARG;
+  std::vector<float4> h((size_t)std::max(ctx->n_sites, 1));
+  if (ctx->n_sites)
+    VD_HIP(hipMemcpyAsync(h.data(), ctx->sites.p, sizeof(float4) * ctx->n_sites, hipMemcpyDeviceToHost, ctx->stream),
+           "read sites");
+  VD_HIP(hipStreamSynchronize(ctx->stream), "read sites");
+  for (int i = 0; i < ctx->n_sites; ++i) {
+    out[3 * i] = h[i].x;
+    out[3 * i + 1] = h[i].y;
+    out[3 * i + 2] = h[i].z;
+  }
+  return VD_OK;
+}
+
+int vd_get_dims(vd_ctx* ctx, int* k, int* tk, int* n_sites, int* n_tets) {
+  if (!ctx) return VD_ERR_ARG;
+  const int K = std::max(0, std::min(ctx->cfg.site_k, ctx->n_sites - 1));
+  const int TK = std::max(0, std::min(tet_k_heuristic(ctx->n_sites, ctx->n_tets, ctx->cfg.tet_k), ctx->n_sites));
+  if (k) *k = K;
+  if (tk) *tk = TK;
+  if (n_sites) *n_sites = ctx->n_sites;
+  if (n_tets) *n_tets = ctx->n_tets;
+  return VD_OK;
+}
+
+int vd_knn_step(vd_ctx* ctx) {
+  if (!ctx) return VD_ERR_ARG;
+  if (ctx->n_sites <= 0) return ctx->fail(VD_ERR_STATE, "vd_knn_step: no sites");
+  VD_HIP(hipSetDevice(ctx->device), "set device");
+  vd_get_dims(ctx, &ctx->K, &ctx->TK, nullptr, nullptr);
+  const int n = ctx->n_sites;
+  VD_HIP(ctx->site_knn.ensure((size_t)n * std::max(ctx->K, 1)), "alloc site knn");
+  VD_HIP(ctx->tet_knn.ensure((size_t)std::max(ctx->n_tets, 1) * std::max(ctx->TK, 1)), "alloc tet knn");
+  VD_HIP(hipEventRecord(ctx->E(0), ctx->stream), "event");
+  const vd::GridBufs gb = grid_bufs(ctx);
+  VD_HIP(vd::grid_build(gb, ctx->sites.p, n, ctx->n_verts ? ctx->mesh_lo : nullptr,
+                        ctx->n_verts ? ctx->mesh_hi : nullptr, ctx->cfg.grid_density, ctx->stream),
+         "grid build");
+  if (ctx->K > 0)
+    VD_HIP(vd::knn_query(gb, n, ctx->offsets.p, ctx->n_offsets, ctx->K, 0, n, nullptr, nullptr, nullptr,
+                         ctx->site_knn.p, 0, ctx->stream),
+           "site knn");
+  if (ctx->n_tets > 0 && ctx->TK > 0)
+    VD_HIP(vd::knn_query(gb, n, ctx->offsets.p, ctx->n_offsets, ctx->TK, 1, ctx->n_tets, ctx->verts.p, ctx->tets.p,
+                         nullptr, ctx->tet_knn.p, 0, ctx->stream),
+           "tet knn");
+  VD_HIP(hipEventRecord(ctx->E(1), ctx->stream), "event");
+  ctx->ev_knn = true;
+  ctx->ev_pairs = ctx->ev_clip = ctx->ev_update = false;
+  ctx->have_knn = true;
+  ctx->have_clip = false;
+  return VD_OK;
+}
+
+int vd_clip_step(vd_ctx* ctx) {
+  if (!ctx) return VD_ERR_ARG;
+  if (!ctx->have_knn) return ctx->fail(VD_ERR_STATE, "vd_clip_step: run vd_knn_step first");
+  VD_HIP(hipSetDevice(ctx->device), "set device");
+  const vd_config& c = ctx->cfg;
+  const int n = ctx->n_sites;
+  const long long n_pairs = (long long)ctx->n_tets * ctx->TK;
+  if (n_pairs > 0x7fffffffLL) return ctx->fail(VD_ERR_ARG, "more than 2^31 pairs in one context");
+  if (c.accum == VD_ACCUM_F32) {
+    VD_HIP(ctx->acc_f32.ensure(4 * (size_t)n), "alloc acc");
+    VD_HIP(vd::fill_u32(ctx->acc_f32.p, 4LL * n, 0u, ctx->stream), "clear acc");
+  } else {
+    VD_HIP(ctx->acc_fix.ensure(4 * (size_t)n), "alloc acc");
+    VD_HIP(vd::fill_u32(ctx->acc_fix.p, 8LL * n, 0u, ctx->stream), "clear acc");
+  }
+  VD_HIP(vd::fill_u32(ctx->hist.p, 16, 0u, ctx->stream), "clear hist");
+  if (c.record_pairs) {
+    VD_HIP(ctx->pair_status.ensure(std::max(n_pairs, 1LL)), "alloc pair status");
+    VD_HIP(ctx->pair_vals.ensure(4 * std::max(n_pairs, 1LL)), "alloc pair vals");
+  }
+  vd::ClipArgs a{};
+  a.sites = ctx->sites.p;
+  a.site_knn = ctx->site_knn.p;
+  a.K = ctx->K;
+  a.verts = ctx->verts.p;
+  a.tets = ctx->tets.p;
+  a.tet_knn = ctx->tet_knn.p;
+  a.TK = ctx->TK;
+  a.n_pairs = n_pairs;
+  a.n_sites = n;
+  a.pcap = c.pcap;
+  a.tcap = c.tcap;
+  a.acc_f32 = ctx->acc_f32.p;
+  a.acc_fix = ctx->acc_fix.p;
+  a.pair_status = c.record_pairs ? ctx->pair_status.p : nullptr;
+  a.pair_vals = c.record_pairs ? ctx->pair_vals.p : nullptr;
+  a.status_hist = ctx->hist.p;
+  int blocks = 0;
+  if (c.form == VD_FORM_SITE_MAJOR) {
+    VD_HIP(ctx->p_count.ensure(n), "alloc pairs");
+    VD_HIP(ctx->p_start.ensure((size_t)n + 1), "alloc pairs");
+    VD_HIP(ctx->p_cursor.ensure(n), "alloc pairs");
+    VD_HIP(ctx->p_ids.ensure(std::max(n_pairs, 1LL)), "alloc pairs");
+    VD_HIP(vd::fill_u32(ctx->p_count.p, n, 0u, ctx->stream), "clear pairs");
+    VD_HIP(vd::pairs_by_site(ctx->tet_knn.p, n_pairs, n, ctx->p_count.p, ctx->p_start.p, ctx->p_cursor.p,
+                             ctx->p_ids.p, ctx->stream),
+           "pairs by site");
+    VD_HIP(hipEventRecord(ctx->E(2), ctx->stream), "event");
+    ctx->ev_pairs = true;
+    a.pair_start = ctx->p_start.p;
+    a.pair_ids = ctx->p_ids.p;
+    blocks = c.clip_blocks > 0 ? c.clip_blocks : n;
+  } else {
+    VD_HIP(hipEventRecord(ctx->E(2), ctx->stream), "event");
+    ctx->ev_pairs = true;
+  }
+  VD_HIP(vd::clip_launch(a, c.form, c.moments, c.accum, blocks, ctx->stream), "clip");
+  VD_HIP(hipEventRecord(ctx->E(3), ctx->stream), "event");
+  ctx->ev_clip = true;
+  ctx->have_clip = true;
+  return VD_OK;
+}
+
+int vd_update_step(vd_ctx* ctx) {
+  if (!ctx) return VD_ERR_ARG;
+  if (!ctx->have_clip) return ctx->fail(VD_ERR_STATE, "vd_update_step: run vd_clip_step first");
+  VD_HIP(vd::lloyd_update(ctx->sites.p, ctx->n_sites, ctx->cfg.accum, ctx->acc_f32.p, ctx->acc_fix.p, ctx->stream),
+         "update");
+  VD_HIP(hipEventRecord(ctx->E(4), ctx->stream), "event");
+  ctx->ev_update = true;
+  ctx->it_slot = (ctx->it_slot + 1) % vd_ctx::kRing;
+  ctx->it_done++;
+  ctx->ev_knn = ctx->ev_pairs = ctx->ev_clip = false;
+  ctx->have_knn = false;  // sites moved
+  return VD_OK;
+}
+
+int vd_lloyd(vd_ctx* ctx, int n_iter) {
+  if (!ctx || n_iter < 0) return VD_ERR_ARG;
+  for (int i = 0; i < n_iter; ++i) {
+    int rc = vd_knn_step(ctx);
+    if (rc == VD_OK) rc = vd_clip_step(ctx);
+    if (rc == VD_OK) rc = vd_update_step(ctx);
+    if (rc != VD_OK) return rc;
+  }
+  return VD_OK;
+}
+
+int vd_sync(vd_ctx* ctx) {
+  if (!ctx) return VD_ERR_ARG;
+  VD_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  return VD_OK;
+}
+
+namespace {
+void slot_timing(vd_ctx* ctx, int slot, bool knn, bool pairs, bool clip, bool update, vd_timing* t) {
+  auto el = [&](int a, int b) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev[a][slot], ctx->ev[b][slot]);
+    return ms;
+  };
+  if (knn) t->knn_ms += el(0, 1);
+  if (knn && pairs) t->pairs_ms += el(1, 2);
+  if (pairs && clip) t->clip_ms += el(2, 3);
+  if (clip && update) t->update_ms += el(3, 4);
+  if (knn && update) t->total_ms += el(0, 4);
+}
+}  // namespace
+
+int vd_get_timing(vd_ctx* ctx, vd_timing* t) {
+  if (!ctx || !t) return VD_ERR_ARG;
+  VD_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  std::memset(t, 0, sizeof *t);
+  // the iteration in flight (no update yet) or else the last completed one
+  int slot = ctx->it_slot;
+  bool upd = false;
+  if (!ctx->ev_knn && ctx->it_done > 0) {
+    slot = (ctx->it_slot + vd_ctx::kRing - 1) % vd_ctx::kRing;
+    upd = true;
+    slot_timing(ctx, slot, true, true, true, true, t);
+    return VD_OK;
+  }
+  (void)upd;
+  slot_timing(ctx, slot, ctx->ev_knn, ctx->ev_pairs, ctx->ev_clip, false, t);
+  return VD_OK;
+}
+
+int vd_timing_reset(vd_ctx* ctx) {
+  if (!ctx) return VD_ERR_ARG;
+  VD_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  ctx->it_done = 0;
+  return VD_OK;
+}
+
+int vd_get_timing_sum(vd_ctx* ctx, vd_timing* t, int* n_iters) {
+  if (!ctx || !t) return VD_ERR_ARG;
+  VD_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  std::memset(t, 0, sizeof *t);
+  const int n = (int)std::min<long long>(ctx->it_done, vd_ctx::kRing);
+  for (int i = 1; i <= n; ++i) slot_timing(ctx, (ctx->it_slot + vd_ctx::kRing - i) % vd_ctx::kRing, true, true, true, true, t);
+  if (n_iters) *n_iters = n;
+  return VD_OK;
+}
+
+int vd_get_site_knn(vd_ctx* ctx, int32_t* out) {
+  if (!ctx || !out) return VD_ERR_ARG;
+  if (!ctx->have_knn && !ctx->have_clip) return ctx->fail(VD_ERR_STATE, "no kNN result");
+  if ((size_t)ctx->n_sites * ctx->K)
+    VD_HIP(hipMemcpyAsync(out, ctx->site_knn.p, sizeof(int32_t) * (size_t)ctx->n_sites * ctx->K,
+                          hipMemcpyDeviceToHost, ctx->stream), "read site knn");
+  VD_HIP(hipStreamSynchronize(ctx->stream), "read site knn");
+  return VD_OK;
+}
+
+int vd_get_tet_knn(vd_ctx* ctx, int32_t* out) {
+  if (!ctx || !out) return VD_ERR_ARG;
+  if (!ctx->have_knn && !ctx->have_clip) return ctx->fail(VD_ERR_STATE, "no kNN result");
+  if ((size_t)ctx->n_tets * ctx->TK)
+    VD_HIP(hipMemcpyAsync(out, ctx->tet_knn.p, sizeof(int32_t) * (size_t)ctx->n_tets * ctx->TK,
+                          hipMemcpyDeviceToHost, ctx->stream), "read tet knn");
+  VD_HIP(hipStreamSynchronize(ctx->stream), "read tet knn");
+  return VD_OK;
+}
+
+int vd_get_moments(vd_ctx* ctx, double* out) {
+  if (!ctx || !out) return VD_ERR_ARG;
+  if (!ctx->have_clip && !ctx->ev_update) return ctx->fail(VD_ERR_STATE, "no clip result");
+  const int n = ctx->n_sites;
+  VD_HIP(ctx->moments.ensure(4 * (size_t)std::max(n, 1)), "alloc moments");
+  VD_HIP(vd::moments_out(n, ctx->cfg.accum, ctx->acc_f32.p, ctx->acc_fix.p, ctx->moments.p, ctx->stream), "moments");
+  VD_HIP(hipMemcpyAsync(out, ctx->moments.p, sizeof(double) * 4 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream),
+         "read moments");
+  VD_HIP(hipStreamSynchronize(ctx->stream), "read moments");
+  return VD_OK;
+}
+
+int vd_get_pairs(vd_ctx* ctx, uint8_t* status, double* vals) {
+  if (!ctx) return VD_ERR_ARG;
+  if (!ctx->cfg.record_pairs) return ctx->fail(VD_ERR_STATE, "record_pairs is off");
+  if (!ctx->have_clip && !ctx->ev_update) return ctx->fail(VD_ERR_STATE, "no clip result");
+  const size_t np = (size_t)ctx->n_tets * ctx->TK;
+  if (status && np)
+    VD_HIP(hipMemcpyAsync(status, ctx->pair_status.p, np, hipMemcpyDeviceToHost, ctx->stream), "read pairs");
+  if (vals && np)
+    VD_HIP(hipMemcpyAsync(vals, ctx->pair_vals.p, 4 * sizeof(double) * np, hipMemcpyDeviceToHost, ctx->stream),
+           "read pairs");
+  VD_HIP(hipStreamSynchronize(ctx->stream), "read pairs");
+  return VD_OK;
+}
+
+int vd_get_status_hist(vd_ctx* ctx, uint64_t* hist7) {
+  if (!ctx || !hist7) return VD_ERR_ARG;
+  unsigned long long h[8];
+  VD_HIP(hipMemcpyAsync(h, ctx->hist.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream), "read hist");
+  VD_HIP(hipStreamSynchronize(ctx->stream), "read hist");
+  for (int i = 0; i < 7; ++i) hist7[i] = h[i];
+  return VD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,542 @@
+// Exact k-nearest-neighbour search on gfx950.
+//
+// Replaces the reference's grid kNN (knearests.cu:19-145, 230-386) and its
+// brute-force kNN (extern/kNN-CUDA/knncuda.cu:20-91, 143-189, 271-349).
+//
+// Result contract: for every query, the k reference points with the smallest
+// (d2, index) keys, nearest first, where d2 = fma(dz,dz, fma(dy,dy, dx*dx)) in
+// FP32 with d = ref - query.  The output does not depend on the grid, the
+// launch shape or the order of work, so it is bit-exact against any exact
+// implementation that uses the same key.
+//
+// Grid: a uniform grid over the bounding box of the reference points (and an
+// optional extra box), sized on the device so that a whole Lloyd iteration can
+// be captured in a graph.  Binning is a counting sort (count, scan, scatter)
+// followed by an in-cell sort by index, so the sorted order is deterministic.
+//
+// Query: one wavefront per query.  The wave walks a table of cell offsets
+// ordered by a query-independent lower bound of their distance, loads the
+// points of 64 cells at a time into its 64 lanes, filters them against the
+// current k-th key, stages survivors in LDS and merges every 64 of them into a
+// sorted top-k held in registers (64*M slots, bitonic sort + bitonic merge
+// across lanes).  It stops when the lower bound of the next offsets exceeds
+// the k-th distance; offsets beyond the table fall back to a full scan.
+#include "vd_knn.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace vd {
+
+namespace {
+
+constexpr int kScanThreads = 1024;
+
+__device__ __forceinline__ int cell_coord(float x, float lo, float inv_h, int dim) {
+  int c = (int)floorf((x - lo) * inv_h);
+  return min(max(c, 0), dim - 1);
+}
+
+__device__ __forceinline__ uint32_t bperm32(uint32_t v, int src_lane) {
+  return (uint32_t)__builtin_amdgcn_ds_bpermute(src_lane << 2, (int)v);
+}
+__device__ __forceinline__ uint64_t bperm64(uint64_t v, int src_lane) {
+  const uint32_t lo = bperm32((uint32_t)v, src_lane);
+  const uint32_t hi = bperm32((uint32_t)(v >> 32), src_lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+__device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a < b ? b : a; }
+
+__device__ __forceinline__ uint64_t knn_key(float d2, uint32_t id) {
+  return ((uint64_t)__float_as_uint(d2) << 32) | id;
+}
+__device__ __forceinline__ float key_d2(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+
+__device__ __forceinline__ float dist2(const float4& r, const float4& q) {
+  const float dx = r.x - q.x, dy = r.y - q.y, dz = r.z - q.z;
+  return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
+// ---------------------------------------------------------------- grid ----
+
+// Bounding box of the points (and of `extra`, a box given by the host), then
+// the grid shape: about n/density cells of equal size, at most max_cells.
+__global__ __launch_bounds__(kScanThreads) void k_grid_shape(const float4* pts, int n, float3 extra_lo,
+                                                              float3 extra_hi, int use_extra, float density,
+                                                              int max_cells, GridShape* out) {
+  __shared__ float red[6][kScanThreads / 64];
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const float4 p = pts[i];
+    lo[0] = fminf(lo[0], p.x); lo[1] = fminf(lo[1], p.y); lo[2] = fminf(lo[2], p.z);
+    hi[0] = fmaxf(hi[0], p.x); hi[1] = fmaxf(hi[1], p.y); hi[2] = fmaxf(hi[2], p.z);
+  }
+  for (int off = 32; off > 0; off >>= 1)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], off));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off));
+    }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int a = 0; a < 3; ++a) {
+      red[a][w] = lo[a];
+      red[3 + a][w] = hi[a];
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 1; j < (int)(blockDim.x >> 6); ++j)
+      for (int a = 0; a < 3; ++a) {
+        red[a][0] = fminf(red[a][0], red[a][j]);
+        red[3 + a][0] = fmaxf(red[3 + a][0], red[3 + a][j]);
+      }
+    double l[3], h[3];
+    const float elo[3] = {extra_lo.x, extra_lo.y, extra_lo.z}, ehi[3] = {extra_hi.x, extra_hi.y, extra_hi.z};
+    for (int a = 0; a < 3; ++a) {
+      l[a] = red[a][0];
+      h[a] = red[3 + a][0];
+      if (use_extra) {
+        l[a] = fmin(l[a], (double)elo[a]);
+        h[a] = fmax(h[a], (double)ehi[a]);
+      }
+      if (!(l[a] <= h[a])) { l[a] = 0.0; h[a] = 0.0; }
+    }
+    double ext = fmax(fmax(h[0] - l[0], h[1] - l[1]), fmax(h[2] - l[2], 1e-20));
+    // pad the box so that no point sits on the upper face
+    for (int a = 0; a < 3; ++a) {
+      l[a] -= 1e-4 * ext;
+      h[a] += 1e-4 * ext;
+    }
+    ext *= 1.0002;
+    double vol = 1.0;
+    for (int a = 0; a < 3; ++a) vol *= fmax(h[a] - l[a], ext * 1e-3);
+    const double cells = fmax(1.0, (double)n / (double)density);
+    double cs = cbrt(vol / cells);
+    int dim[3];
+    for (;;) {
+      long long tot = 1;
+      for (int a = 0; a < 3; ++a) {
+        dim[a] = (int)fmin(1024.0, fmax(1.0, ceil((h[a] - l[a]) / cs)));
+        tot *= dim[a];
+      }
+      if (tot <= max_cells) break;
+      cs *= 1.05;
+    }
+    GridShape g;
+    for (int a = 0; a < 3; ++a) {
+      g.lo[a] = (float)l[a];
+      g.dim[a] = dim[a];
+    }
+    g.h = (float)cs;
+    g.inv_h = (float)(1.0 / cs);
+    g.ncell = dim[0] * dim[1] * dim[2];
+    // absolute slack for the cell-box distance bounds (rounding of the cell
+    // assignment and of lo + c*h)
+    g.slack = (float)(1e-3 * cs + 1e-6 * (fabs(l[0]) + fabs(l[1]) + fabs(l[2]) + 1024.0 * cs));
+    *out = g;
+  }
+}
+
+__global__ void k_grid_count(const float4* pts, int n, const GridShape* gs, int* cell_of, int* count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const GridShape g = *gs;
+  const float4 p = pts[i];
+  const int cx = cell_coord(p.x, g.lo[0], g.inv_h, g.dim[0]);
+  const int cy = cell_coord(p.y, g.lo[1], g.inv_h, g.dim[1]);
+  const int cz = cell_coord(p.z, g.lo[2], g.inv_h, g.dim[2]);
+  const int c = cx + g.dim[0] * (cy + g.dim[1] * cz);
+  cell_of[i] = c;
+  atomicAdd(count + c, 1);
+}
+
+// Exclusive scan of count[0..ncell) into start[0..ncell], one workgroup;
+// also clears the scatter cursors.
+__global__ __launch_bounds__(kScanThreads) void k_grid_scan(const GridShape* gs, const int* count, int* start,
+                                                             int* cursor) {
+  __shared__ int part[kScanThreads];
+  const int ncell = gs->ncell;
+  const int per = (ncell + kScanThreads - 1) / kScanThreads;
+  const int b = threadIdx.x * per, e = min(ncell, b + per);
+  int s = 0;
+  for (int i = b; i < e; ++i) s += count[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+  for (int i = b; i < e; ++i) {
+    start[i] = run;
+    run += count[i];
+    cursor[i] = 0;
+  }
+  if (threadIdx.x == kScanThreads - 1) start[ncell] = part[kScanThreads - 1];
+}
+
+__global__ void k_grid_scatter(int n, const int* cell_of, const int* start, int* cursor, int* sorted_ids) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell_of[i];
+  sorted_ids[start[c] + atomicAdd(cursor + c, 1)] = i;
+}
+
+// One thread per cell: order the cell's ids, then gather the points.
+__global__ void k_grid_sort(const GridShape* gs, const float4* pts, const int* start, int* sorted_ids,
+                            float4* sorted_pts) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= gs->ncell) return;
+  const int b = start[c], e = start[c + 1];
+  for (int i = b + 1; i < e; ++i) {
+    const int v = sorted_ids[i];
+    int j = i - 1;
+    while (j >= b && sorted_ids[j] > v) {
+      sorted_ids[j + 1] = sorted_ids[j];
+      --j;
+    }
+    sorted_ids[j + 1] = v;
+  }
+  for (int i = b; i < e; ++i) sorted_pts[i] = pts[sorted_ids[i]];
+}
+
+// ---------------------------------------------------------------- query ---
+
+struct QueryArgs {
+  const GridShape* grid;
+  const float4* sorted_pts;
+  const int* sorted_ids;
+  const int* start;
+  int n_ref;
+  const int4* offsets;  // (dx, dy, dz, lb) sorted by lb = sum max(|d|-1,0)^2
+  int n_offsets;
+  int k;
+  int n_query;
+  int mode;  // 0: queries are the reference points themselves (self excluded)
+             // 1: queries are tet centroids
+             // 2: queries are given points (self not excluded)
+  const float4* verts;
+  const int4* tets;
+  const float4* qpts;
+  int32_t* out;        // row-major n_query x k, or pitched rows when pitch > 0
+  long long out_pitch; // > 0: out[j * pitch + q] (column per query)
+};
+
+template <int M>
+struct TopK {
+  uint64_t s[M];
+};
+
+// Sorted merge of 64 new keys (one per lane) into the top list.
+template <int M>
+__device__ __forceinline__ void topk_merge(TopK<M>& L, uint64_t x, int lane) {
+  // bitonic sort of x across the wave, ascending
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const uint64_t o = bperm64(x, lane ^ j);
+      const bool up = (lane & k) == 0;
+      const bool lower = (lane & j) == 0;
+      x = (lower == up) ? umin64(x, o) : umax64(x, o);
+    }
+  }
+  // min against the reversed batch gives the 64*M smallest as a bitonic sequence
+  const uint64_t r = bperm64(x, 63 - lane);
+  L.s[M - 1] = umin64(L.s[M - 1], r);
+#pragma unroll
+  for (int js = M / 2; js >= 1; js >>= 1) {
+#pragma unroll
+    for (int s = 0; s < M; ++s) {
+      if ((s & js) == 0) {
+        const uint64_t a = L.s[s], b = L.s[s + js];
+        L.s[s] = umin64(a, b);
+        L.s[s + js] = umax64(a, b);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 32; j > 0; j >>= 1) {
+    const bool lower = (lane & j) == 0;
+#pragma unroll
+    for (int s = 0; s < M; ++s) {
+      const uint64_t o = bperm64(L.s[s], lane ^ j);
+      L.s[s] = lower ? umin64(L.s[s], o) : umax64(L.s[s], o);
+    }
+  }
+}
+
+template <int M>
+__device__ __forceinline__ uint64_t topk_kth(const TopK<M>& L, int k) {
+  const int slot = (k - 1) >> 6, ln = (k - 1) & 63;
+  uint64_t v = 0;
+#pragma unroll
+  for (int s = 0; s < M; ++s)
+    if (s == slot) v = L.s[s];
+  const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, ln);
+  const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), ln);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Squared distance from q to the box of cell (cx,cy,cz), minus the slack,
+// scaled down so that any point binned in that cell has a larger FP32 d2.
+__device__ __forceinline__ float cell_bound(const GridShape& g, const float4& q, int cx, int cy, int cz) {
+  const int c[3] = {cx, cy, cz};
+  const float qq[3] = {q.x, q.y, q.z};
+  float s = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float lo = g.lo[a] + (float)c[a] * g.h;
+    const float hi = lo + g.h;
+    float d = 0.f;
+    if (qq[a] < lo) d = lo - qq[a];
+    if (qq[a] > hi) d = qq[a] - hi;
+    d = fmaxf(0.f, d - g.slack);
+    s = fmaf(d, d, s);
+  }
+  return s * (1.f - 1e-5f);
+}
+
+constexpr int kQueryWaves = 4;
+
+template <int M>
+__global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
+  __shared__ uint64_t stage_all[kQueryWaves][128];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  uint64_t* stage = stage_all[wv];
+  const int q = blockIdx.x * kQueryWaves + wv;
+  if (q >= a.n_query) return;  // whole wave leaves together
+  const GridShape g = *a.grid;
+
+  float4 qp;
+  int self = -1;
+  int out_row = q;
+  if (a.mode == 0) {
+    qp = a.sorted_pts[q];
+    self = a.sorted_ids[q];
+    out_row = self;
+  } else if (a.mode == 1) {
+    const int4 t = a.tets[q];
+    const float4 v0 = a.verts[t.x], v1 = a.verts[t.y], v2 = a.verts[t.z], v3 = a.verts[t.w];
+    qp.x = (((v0.x + v1.x) + v2.x) + v3.x) * 0.25f;
+    qp.y = (((v0.y + v1.y) + v2.y) + v3.y) * 0.25f;
+    qp.z = (((v0.z + v1.z) + v2.z) + v3.z) * 0.25f;
+    qp.w = 0.f;
+  } else {
+    qp = a.qpts[q];
+  }
+  const int qc[3] = {cell_coord(qp.x, g.lo[0], g.inv_h, g.dim[0]), cell_coord(qp.y, g.lo[1], g.inv_h, g.dim[1]),
+                     cell_coord(qp.z, g.lo[2], g.inv_h, g.dim[2])};
+
+  TopK<M> L;
+#pragma unroll
+  for (int s = 0; s < M; ++s) L.s[s] = ~0ull;
+  uint64_t thr = ~0ull;
+  int staged = 0;
+  bool done = false;
+
+  auto push = [&](uint64_t key, bool ok) {
+    const uint64_t mask = __ballot(ok);
+    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+    if (ok) stage[staged + below] = key;
+    staged += __popcll(mask);
+    wave_sync();
+    if (staged >= 64) {
+      const uint64_t x = stage[lane];
+      const uint64_t rest = (lane + 64 < staged) ? stage[lane + 64] : ~0ull;
+      wave_sync();
+      if (lane + 64 < staged) stage[lane] = rest;
+      staged -= 64;
+      wave_sync();
+      topk_merge<M>(L, x, lane);
+      thr = topk_kth<M>(L, a.k);
+    }
+  };
+
+  for (int ob = 0; ob < a.n_offsets && !done; ob += 64) {
+    const float kth = key_d2(thr);
+    // offsets are sorted by lb: once the first of the batch is beyond the k-th
+    // distance, so are all later ones
+    const float lbd = fmaxf(0.f, sqrtf((float)a.offsets[ob].w) * g.h - 2.f * g.slack);
+    if (thr != ~0ull && lbd * lbd * (1.f - 1e-5f) > kth) {
+      done = true;
+      break;
+    }
+    int cnt = 0, st = 0;
+    const int oi = ob + lane;
+    if (oi < a.n_offsets) {
+      const int4 o = a.offsets[oi];
+      const int cx = qc[0] + o.x, cy = qc[1] + o.y, cz = qc[2] + o.z;
+      if (cx >= 0 && cx < g.dim[0] && cy >= 0 && cy < g.dim[1] && cz >= 0 && cz < g.dim[2]) {
+        if (thr == ~0ull || !(cell_bound(g, qp, cx, cy, cz) > kth)) {
+          const int c = cx + g.dim[0] * (cy + g.dim[1] * cz);
+          st = a.start[c];
+          cnt = a.start[c + 1] - st;
+        }
+      }
+    }
+    // exclusive prefix of the counts across the wave
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int v = __shfl_up(incl, d);
+      if (lane >= d) incl += v;
+    }
+    const int excl = incl - cnt;
+    const int total = __shfl(incl, 63);
+    for (int f0 = 0; f0 < total; f0 += 64) {
+      const int f = f0 + lane;
+      // owner lane of flat index f: the largest lane whose prefix is <= f
+      // (every lane runs the search so that no shuffle reads an idle lane)
+      int own = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1) {
+        const int cand = own + step;
+        const int pe = __shfl(excl, cand & 63);
+        if (cand < 64 && pe <= f) own = cand;
+      }
+      const int o_st = __shfl(st, own), o_ex = __shfl(excl, own);
+      uint64_t key = ~0ull;
+      bool ok = false;
+      if (f < total) {
+        const int idx = o_st + (f - o_ex);
+        const int id = a.sorted_ids[idx];
+        key = knn_key(dist2(a.sorted_pts[idx], qp), (uint32_t)id);
+        ok = (id != self) && key < thr;
+      }
+      push(key, ok);
+    }
+  }
+  if (!done) {
+    // offsets table exhausted without the stop test passing: scan everything
+    for (int f0 = 0; f0 < a.n_ref; f0 += 64) {
+      const int f = f0 + lane;
+      uint64_t key = ~0ull;
+      bool ok = false;
+      if (f < a.n_ref) {
+        // skip cells already visited: those inside the offset table's reach
+        const float4 p = a.sorted_pts[f];
+        const int id = a.sorted_ids[f];
+        const int cx = cell_coord(p.x, g.lo[0], g.inv_h, g.dim[0]) - qc[0];
+        const int cy = cell_coord(p.y, g.lo[1], g.inv_h, g.dim[1]) - qc[1];
+        const int cz = cell_coord(p.z, g.lo[2], g.inv_h, g.dim[2]) - qc[2];
+        const int r = max(abs(cx), max(abs(cy), abs(cz)));
+        key = knn_key(dist2(p, qp), (uint32_t)id);
+        ok = (r > kOffsetRadius) && (id != self) && key < thr;
+      }
+      push(key, ok);
+    }
+  }
+  if (staged > 0) {
+    const uint64_t x = lane < staged ? stage[lane] : ~0ull;
+    topk_merge<M>(L, x, lane);
+  }
+#pragma unroll
+  for (int s = 0; s < M; ++s) {
+    const int j = s * 64 + lane;
+    if (j < a.k) {
+      const int32_t id = L.s[s] == ~0ull ? -1 : (int32_t)(uint32_t)L.s[s];
+      if (a.out_pitch > 0)
+        a.out[(long long)j * a.out_pitch + out_row] = id;
+      else
+        a.out[(long long)out_row * a.k + j] = id;
+    }
+  }
+}
+
+}  // namespace
+
+__global__ void k_fill_u32(uint32_t* p, long long n, uint32_t v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+hipError_t fill_u32(void* p, long long n, uint32_t v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const long long blocks = std::min<long long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, st, (uint32_t*)p, n, v);
+  return hipGetLastError();
+}
+
+std::vector<int4> knn_offset_table() {
+  std::vector<int4> t;
+  const int R = kOffsetRadius;
+  for (int z = -R; z <= R; ++z)
+    for (int y = -R; y <= R; ++y)
+      for (int x = -R; x <= R; ++x) {
+        auto m = [](int d) { return std::max(std::abs(d) - 1, 0); };
+        t.push_back(make_int4(x, y, z, m(x) * m(x) + m(y) * m(y) + m(z) * m(z)));
+      }
+  std::stable_sort(t.begin(), t.end(), [](const int4& a, const int4& b) {
+    if (a.w != b.w) return a.w < b.w;
+    const int ca = std::max(std::abs(a.x), std::max(std::abs(a.y), std::abs(a.z)));
+    const int cb = std::max(std::abs(b.x), std::max(std::abs(b.y), std::abs(b.z)));
+    return ca < cb;
+  });
+  return t;
+}
+
+int grid_max_cells(int n, float density) {
+  const double c = std::max(1.0, n / (double)density);
+  return (int)std::min(64.0 * 1024 * 1024, std::max(64.0, 1.25 * c + 64.0));
+}
+
+hipError_t grid_build(const GridBufs& gb, const float4* pts, int n, const float* extra_lo, const float* extra_hi,
+                      float density, hipStream_t st) {
+  const float3 elo = extra_lo ? make_float3(extra_lo[0], extra_lo[1], extra_lo[2]) : make_float3(0, 0, 0);
+  const float3 ehi = extra_hi ? make_float3(extra_hi[0], extra_hi[1], extra_hi[2]) : make_float3(0, 0, 0);
+  hipLaunchKernelGGL(k_grid_shape, dim3(1), dim3(kScanThreads), 0, st, pts, n, elo, ehi, extra_lo ? 1 : 0, density,
+                     gb.max_cells, gb.shape);
+  hipError_t e = fill_u32(gb.count, gb.max_cells, 0u, st);
+  if (e != hipSuccess) return e;
+  const int nb = (n + 255) / 256;
+  if (n > 0) hipLaunchKernelGGL(k_grid_count, dim3(nb), dim3(256), 0, st, pts, n, gb.shape, gb.cell_of, gb.count);
+  hipLaunchKernelGGL(k_grid_scan, dim3(1), dim3(kScanThreads), 0, st, gb.shape, gb.count, gb.start, gb.cursor);
+  if (n > 0)
+    hipLaunchKernelGGL(k_grid_scatter, dim3(nb), dim3(256), 0, st, n, gb.cell_of, gb.start, gb.cursor,
+                       gb.sorted_ids);
+  hipLaunchKernelGGL(k_grid_sort, dim3((gb.max_cells + 255) / 256), dim3(256), 0, st, gb.shape, pts, gb.start,
+                     gb.sorted_ids, gb.sorted_pts);
+  return hipGetLastError();
+}
+
+hipError_t knn_query(const GridBufs& gb, int n_ref, const int4* offsets, int n_offsets, int k, int mode,
+                     int n_query, const float4* verts, const int4* tets, const float4* qpts, int32_t* out,
+                     long long out_pitch, hipStream_t st) {
+  if (k < 1 || k > 256) return hipErrorInvalidValue;
+  QueryArgs a;
+  a.grid = gb.shape;
+  a.sorted_pts = gb.sorted_pts;
+  a.sorted_ids = gb.sorted_ids;
+  a.start = gb.start;
+  a.n_ref = n_ref;
+  a.offsets = offsets;
+  a.n_offsets = n_offsets;
+  a.k = k;
+  a.n_query = n_query;
+  a.mode = mode;
+  a.verts = verts;
+  a.tets = tets;
+  a.qpts = qpts;
+  a.out = out;
+  a.out_pitch = out_pitch;
+  if (n_query <= 0) return hipSuccess;
+  const dim3 grid((n_query + kQueryWaves - 1) / kQueryWaves), block(64 * kQueryWaves);
+  if (k <= 64)
+    hipLaunchKernelGGL(k_knn_query<1>, grid, block, 0, st, a);
+  else if (k <= 128)
+    hipLaunchKernelGGL(k_knn_query<2>, grid, block, 0, st, a);
+  else
+    hipLaunchKernelGGL(k_knn_query<4>, grid, block, 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace vd

@@ -1,0 +1,193 @@
+"""Lloyd-iteration throughput of the clipped-Voronoi engine on one MI355X.
+
+Workload (the single-solve configuration): unit cube split into 55^3 cubes x 6
+Kuhn tets (998,250 tets, vertices jittered by U(-0.1,0.1)*h), 100,000 uniform
+FP32 sites, candidate sites per tet TK = 64, neighbours per site K = 90.  A
+step is one whole Lloyd iteration on the device: grid kNN of the sites,
+candidate kNN of the tets, clipping of all n_tets*TK (tet, site) pairs with
+volume/moment accumulation, site update.  Inputs are resident in HBM before
+the timed region.
+
+Reported:
+  value           Lloyd iterations/s of the engine's default fast form
+                  (site-major launch, FP32 clipping, FP64 accumulation)
+  plain_*         the plain form: tet-major launch (one lane per pair, the
+                  reference's order), FP32 accumulation with FP32 atomics
+  roofline        clipping kernel: algorithmic FLOPs per launch / measured
+                  kernel time vs the FP32 vector peak
+  cpu_baseline    the oracle's FP32 restatement of the reference's kernel A
+                  on the host cores, on a bounded sample of the same pairs
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "3d-voronoi-gpu_amd"))
+
+METRIC = "Lloyd iters/s at 1M tets,100k sites,k=64; batched/served p50/p99 latency"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+N_CUBE, N_SITES, TK, K = 55, 100000, 64, 90
+
+# FLOP weights per event of the reference's kernel A (add/mul = 1, fma = 2):
+# pair setup (4 face planes), bisector, conflict test (det4x4), security-radius
+# term per vertex (vertex position + distance), per security check, moment
+# integration per vertex.
+W_PAIR, W_CLIP, W_CONFLICT, W_SECURE_VTX, W_SECURE_CHECK, W_ACC_VTX = 80, 14, 45, 67, 9, 545
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_engine(native, verts, tets, sites, steps, warmup, **cfg):
+    e = native.Engine(0, site_k=K, tet_k=TK, **cfg)
+    e.set_mesh(verts, tets)
+    e.set_sites(sites)
+    for _ in range(warmup):
+        e.lloyd(1)
+    e.sync()
+    e.timing_reset()
+    t0 = time.perf_counter()
+    e.lloyd(steps)
+    e.sync()
+    wall = time.perf_counter() - t0
+    tsum = e.timing_sum()
+    hist = e.status_hist()
+    e.close()
+    return wall, tsum, hist
+
+
+def cpu_baseline(verts, tets, sites, budget_s):
+    """Oracle FP32 kernel A on a bounded, evenly spread sample of tets."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    threads = O.nthreads()
+    t0 = time.perf_counter()
+    sk = O.site_knn(sites, K)
+    t_site_knn = time.perf_counter() - t0
+    # candidate kNN and clipping on a sample; first a small probe to size it
+    sample = []
+    for frac in (1 / 512.0, None):
+        if frac is None:
+            per_tet = probe_s / len(probe)
+            n = int(min(len(tets), max(len(probe), 0.5 * budget_s / per_tet)))
+        else:
+            n = max(64, int(len(tets) * frac))
+        sel = np.linspace(0, len(tets) - 1, n).astype(np.int64)
+        t0 = time.perf_counter()
+        cent = O.tet_centroids(verts, tets[sel])
+        tk = O.knn(sites, cent, TK, False)
+        t_knn = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r = O.clip_tets(sites, sk, verts, tets[sel], tk, geom="f32", pair_acc="f32", site_mode=O.SITE_F32,
+                        want_pairs=False)
+        t_clip = time.perf_counter() - t0
+        if frac is not None:
+            probe, probe_s = sel, t_clip + t_knn
+        else:
+            sample = (sel, t_knn, t_clip, r["events"])
+    sel, t_knn, t_clip, ev = sample
+    scale = len(tets) / len(sel)
+    t_iter = t_site_knn + (t_knn + t_clip) * scale
+    return dict(
+        iters_per_s=1.0 / t_iter,
+        cores=threads,
+        sample=f"{len(sel)} of {len(tets)} tets (evenly spaced) x TK={TK} pairs; site kNN of all {len(sites)} sites; "
+               f"clip {t_clip:.2f}s + candidate kNN {t_knn:.2f}s + site kNN {t_site_knn:.2f}s, scaled to the full mesh",
+        events=ev,
+        sample_pairs=len(sel) * TK,
+        clip_s_per_pair=t_clip / (len(sel) * TK),
+    )
+
+
+def flops_per_pair(ev):
+    f = (W_PAIR * ev["pairs"] + W_CLIP * ev["clips"] + W_CONFLICT * ev["conflict_tests"]
+         + W_SECURE_VTX * ev["secure_evals"] + W_SECURE_CHECK * ev["secure_checks"] + W_ACC_VTX * ev["acc_tris"])
+    return f / max(ev["pairs"], 1)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU work for the baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        sys.exit(2)
+
+    from voronoi_amd import meshgen, native
+
+    t0 = time.perf_counter()
+    verts, tets, sites = meshgen.make_problem("cube", N_CUBE, N_SITES, seed=7)
+    log(f"workload: {len(tets)} tets, {len(verts)} verts, {len(sites)} sites ({time.perf_counter() - t0:.1f}s)")
+
+    wall, ts, hist = run_engine(native, verts, tets, sites, args.steps, args.warmup, form="site", accum="f64")
+    n = ts["iters"]
+    ms_step = 1e3 * wall / args.steps
+    log(f"fast: {ms_step:.2f} ms/iter wall; per-iter device: knn {ts['knn_ms']/n:.2f} pairs {ts['pairs_ms']/n:.2f} "
+        f"clip {ts['clip_ms']/n:.2f} update {ts['update_ms']/n:.3f} ms; status {hist}")
+
+    pwall, pts, _ = run_engine(native, verts, tets, sites, args.plain_steps, 1, form="tet", accum="f32")
+    pn = pts["iters"]
+    log(f"plain: {1e3 * pwall / args.plain_steps:.2f} ms/iter wall; clip {pts['clip_ms']/pn:.2f} ms")
+
+    n_pairs = len(tets) * TK
+    cpu = None
+    fpp = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(verts, tets, sites, args.cpu_budget)
+        fpp = flops_per_pair(cpu["events"])
+        log(f"cpu baseline: {cpu['iters_per_s']:.4f} iters/s on {cpu['cores']} threads; {fpp:.0f} FLOP/pair")
+
+    clip_s = ts["clip_ms"] / n / 1e3
+    roof = None
+    if fpp is not None:
+        achieved = fpp * n_pairs / clip_s / 1e12
+        roof = {"bound": "valu", "kernel": "k_clip_site (clipping + accumulation)", "achieved": round(achieved, 3),
+                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
+                "traffic": None, "flop_per_launch": fpp * n_pairs, "kernel_ms": round(clip_s * 1e3, 3)}
+    value = args.steps / wall
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "jittered cube 55^3x6 tets (998250), 100000 uniform sites, TK=64, K=90, "
+                               "1 Lloyd iteration per step",
+                   "n_tets": int(len(tets)), "n_sites": int(N_SITES), "tet_k": TK, "site_k": K,
+                   "clip_precision": "fp32", "accumulation": "fp64 (exact fixed-point site sums)",
+                   "launch": "site-major"},
+        "phase_ms": {k: round(ts[k] / n, 3) for k in ("knn_ms", "pairs_ms", "clip_ms", "update_ms", "total_ms")},
+        "plain_fp32_iters_per_s": round(args.plain_steps / pwall, 4),
+        "plain_clip_ms": round(pts["clip_ms"] / pn, 3),
+        "speedup_vs_plain": round(value / (args.plain_steps / pwall), 3),
+        "roofline": roof,
+        "cpu_baseline": None if cpu is None else {
+            "value": round(cpu["iters_per_s"], 5), "unit": "iters/s", "cores": cpu["cores"], "kind": "port",
+            "sample": cpu["sample"]},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

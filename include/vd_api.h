@@ -1,0 +1,183 @@
+/*
+ * vd_api.h -- C ABI of libvoronoi_hip.so, the MI355X (gfx950) clipped-3D-Voronoi
+ * Lloyd engine.
+ *
+ * Every entry point returns an int status (VD_OK or a negative VD_ERR_*) and
+ * never exits the process; vd_ctx_error() gives the message of the last
+ * failure.  A context owns its device buffers and one HIP stream, and is safe
+ * to use from one host thread at a time.  Plain pointers and sizes only.
+ *
+ * Coordinates are those of the reference: the mesh and the sites live in the
+ * [0,1000]^3 frame produced by the reference loader (test_voronoi.cu:14-38,
+ * 103-118); several thresholds of the algorithm are absolute in that frame.
+ *
+ * Which reference interface each entry point replaces is given beside it
+ * (files of xh-liu-tech/3D-Voronoi-GPU).
+ */
+#ifndef VD_API_H
+#define VD_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define VD_OK 0
+#define VD_ERR_ARG (-1)          /* bad argument */
+#define VD_ERR_HIP (-2)          /* HIP runtime error */
+#define VD_ERR_STATE (-3)        /* call out of order (e.g. no mesh yet) */
+#define VD_ERR_NOMEM (-4)        /* device allocation failed */
+#define VD_ERR_PRECONDITION (-5) /* input violates a documented precondition */
+
+/* per-pair status values: identical to the reference's enum Status
+ * (voronoi_defs.h:4-12) */
+#define VD_TRIANGLE_OVERFLOW 0
+#define VD_VERTEX_OVERFLOW 1
+#define VD_INCONSISTENT_BOUNDARY 2
+#define VD_SECURITY_RADIUS_NOT_REACHED 3
+#define VD_SUCCESS 4
+#define VD_NEEDS_EXACT_PREDICATES 5
+#define VD_NO_INTERSECTION 6
+
+/* per-site accumulation */
+#define VD_ACCUM_F32 0 /* FP32 pair sums, FP32 atomics (the reference's, voronoi.cu:577-618) */
+#define VD_ACCUM_F64 1 /* FP64 pair sums, exact 64-bit fixed-point site sums (2^-20), FP64 update */
+
+/* volume / moment decomposition */
+#define VD_MOMENTS_REFERENCE 0 /* six tets per vertex around the seed (voronoi.cu:532-537) */
+#define VD_MOMENTS_ROBUST 1    /* same ring around the vertex mean, edge feet on the edge line */
+
+/* launch form of the clipping kernel (same per-pair results) */
+#define VD_FORM_TET_MAJOR 0  /* one lane per (tet, rank) pair, the reference's order */
+#define VD_FORM_SITE_MAJOR 1 /* one wavefront per site, bisector planes shared in LDS */
+
+typedef struct vd_config {
+  int site_k;         /* neighbours per site; reference _K_ = 90 (params.h:9) */
+  int tet_k;          /* candidate sites per tet; 0 = reference heuristic (voronoi.cu:936) */
+  int pcap;           /* plane cap per polytope; reference _MAX_P_ = 64 (params.h:11) */
+  int tcap;           /* vertex cap per polytope; reference _MAX_T_ = 96 (params.h:12) */
+  int accum;          /* VD_ACCUM_* */
+  int moments;        /* VD_MOMENTS_* */
+  int form;           /* VD_FORM_* */
+  float grid_density; /* sites per kNN grid cell */
+  int record_pairs;   /* keep per-pair status and values for vd_get_pairs */
+  int clip_blocks;    /* workgroups of the site-major launch; 0 = automatic */
+} vd_config;
+
+typedef struct vd_timing {
+  float knn_ms;    /* grid build + site kNN + candidate kNN */
+  float pairs_ms;  /* grouping of pairs by site (site-major form) */
+  float clip_ms;   /* clipping + per-site accumulation */
+  float update_ms; /* Lloyd update */
+  float total_ms;  /* whole iteration */
+} vd_timing;
+
+typedef struct vd_ctx vd_ctx;
+
+/* Defaults: site_k 90, tet_k 0, pcap 64, tcap 96, VD_ACCUM_F32,
+ * VD_MOMENTS_REFERENCE, VD_FORM_SITE_MAJOR, grid_density 3.1
+ * (knearests.cu:243), record_pairs 0, clip_blocks 0. */
+void vd_config_default(vd_config* cfg);
+
+/* Context on HIP device `device`.  Replaces the device selection and buffer
+ * setup of compute_clipped_voro_diagram_GPU (voronoi.cu:933-986). */
+int vd_ctx_create(vd_ctx** out, int device, const vd_config* cfg);
+void vd_ctx_destroy(vd_ctx* ctx);
+const char* vd_ctx_error(const vd_ctx* ctx);
+int vd_ctx_set_config(vd_ctx* ctx, const vd_config* cfg);
+
+/* Mesh: n_verts xyz triples and n_tets quadruples of 0-based vertex ids, every
+ * tet positively oriented (first vertex on the positive side of face c,b,d).
+ * Replaces copy_tet_data (voronoi.cu:881-923). */
+int vd_set_mesh(vd_ctx* ctx, const float* verts_xyz, int n_verts, const int32_t* tets, int n_tets);
+/* Sites, xyz triples in input order (the engine never reorders them). */
+int vd_set_sites(vd_ctx* ctx, const float* sites_xyz, int n_sites);
+int vd_get_sites(vd_ctx* ctx, float* sites_xyz);
+
+/* One Lloyd iteration in three steps, or n_iter whole iterations.
+ * vd_knn_step replaces kn_prepare+kn_solve / knn_cuda_global_dev and
+ * compute_tet_knn_dev (voronoi.cu:995-1028, 825-879); vd_clip_step replaces
+ * kernel A and its accumulation (voronoi.cu:1034-1076); vd_update_step replaces
+ * compute_new_site (voronoi.cu:1121-1132). */
+int vd_knn_step(vd_ctx* ctx);
+int vd_clip_step(vd_ctx* ctx);
+int vd_update_step(vd_ctx* ctx);
+int vd_lloyd(vd_ctx* ctx, int n_iter);
+int vd_sync(vd_ctx* ctx);
+/* Device times (HIP events on the context's stream) of the phases of the
+ * iteration in progress, or of the last completed one. */
+int vd_get_timing(vd_ctx* ctx, vd_timing* t);
+/* Sum of the phase times of the iterations completed since vd_timing_reset
+ * (at most the last 256), and their number. */
+int vd_timing_reset(vd_ctx* ctx);
+int vd_get_timing_sum(vd_ctx* ctx, vd_timing* t, int* n_iters);
+
+/* Effective sizes: K = min(site_k, n_sites-1), TK = min(tet_k or heuristic, n_sites). */
+int vd_get_dims(vd_ctx* ctx, int* k, int* tk, int* n_sites, int* n_tets);
+/* Readback of the last kNN step: n_sites x K neighbour ids (original order,
+ * nearest first, self excluded) and n_tets x TK candidate ids. */
+int vd_get_site_knn(vd_ctx* ctx, int32_t* out);
+int vd_get_tet_knn(vd_ctx* ctx, int32_t* out);
+/* Per-site (volume, Σvol·x, Σvol·y, Σvol·z) of the last clip step. */
+int vd_get_moments(vd_ctx* ctx, double* out);
+/* Per-pair status and (vol, mx, my, mz) of the last clip step; needs
+ * record_pairs.  Pair index = tet * TK + rank. */
+int vd_get_pairs(vd_ctx* ctx, uint8_t* status, double* vals);
+/* Number of pairs per status value (7 counters) of the last clip step. */
+int vd_get_status_hist(vd_ctx* ctx, uint64_t* hist7);
+
+/* ---- kNN entry points of the reference's own surface -------------------- */
+
+/* Exact k nearest neighbours on the device, replacing knn_cuda_global_dev
+ * (extern/kNN-CUDA/knncuda.h:1-12, knncuda.cu:271-349).  ref and query are
+ * dim x n row-pitched device arrays (pitches in elements); the result is
+ * written to knn_index_dev[j * index_pitch + q], j = 0..k-1, nearest first,
+ * ties broken by the lower reference index.  Only dim == 3 is supported.
+ * `stream` is a hipStream_t or NULL. */
+int vd_knn_dev(const float* ref_dev, int ref_nb, size_t ref_pitch, const float* query_dev, int query_nb,
+               size_t query_pitch, int dim, int k, int* knn_index_dev, size_t index_pitch, void* stream);
+
+/* Grid kNN problem, replacing kn_prepare / kn_solve / kn_free and the getters
+ * (knearests.h:3-27, knearests.cu:230-432).  Points are host xyz triples; the
+ * device copy is reordered by grid cell (deterministically: by cell, then by
+ * input index).  vd_kn_solve fills k neighbours per point, self excluded,
+ * as indices into the reordered array, nearest first. */
+typedef struct vd_kn vd_kn;
+typedef struct vd_kn_info {
+  int allocated_points;
+  int dimx, dimy, dimz;
+  int num_cell_offsets;
+  float* d_stored_points;      /* device, reordered xyz triples */
+  unsigned int* d_knearests;   /* device, allocated_points x k */
+  unsigned int* d_permutation; /* device, reordered position -> input index */
+  int* d_counters;             /* device, points per cell */
+  int* d_ptrs;                 /* device, first position per cell */
+} vd_kn_info;
+int vd_kn_prepare(vd_kn** out, const float* points_xyz, int n, int k);
+int vd_kn_solve(vd_kn* kn);
+void vd_kn_free(vd_kn* kn);
+int vd_kn_info_get(const vd_kn* kn, vd_kn_info* info);
+int vd_kn_get_points(vd_kn* kn, float* out_xyz);
+int vd_kn_get_knearests(vd_kn* kn, unsigned int* out);
+int vd_kn_get_permutation(vd_kn* kn, unsigned int* out);
+
+/* ---- whole-program entry point ------------------------------------------ */
+
+/* The reference's Lloyd driver compute_clipped_voro_diagram_GPU
+ * (voronoi.h:78-84, voronoi.cu:925-1188) with plain pointers: vertices are
+ * normalised xyz triples, `site` is updated in place (SoA when
+ * site_is_transposed, else AoS).  Appends one row per iteration to
+ * <out_dir>/record.csv and writes <out_dir>/iter_<i>.xyz, like the reference;
+ * out_dir NULL or "" means the working directory.  cfg NULL = defaults. */
+int vd_compute_clipped_voro_diagram(const float* vertices, int n_verts, const int32_t* indices, int n_tets,
+                                    float* site, int n_site, int site_is_transposed, int nb_lloyd_iter,
+                                    int preferred_tet_k, const vd_config* cfg, const char* out_dir);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VD_API_H */

@@ -1,0 +1,134 @@
+"""Clipping kernels on the GPU against the oracle.
+
+Per-pair results (status, volume, moments) are compared bit for bit with the
+oracle's FP32 restatement of the reference's kernel A: the GPU and the oracle
+evaluate the same expressions with the same fused operations.  Per-site sums
+are bit-exact in the FP64 mode (order-independent fixed-point reduction) and
+compared to 1e-5 relative in the FP32 mode (atomic order varies).
+"""
+import numpy as np
+import pytest
+from conftest import oracle_knn, problem
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("cube", 8, 300), ("cube", 20, 2000), ("torus", 24, 3000)]
+
+
+def _run(vdn, verts, tets, sites, **cfg):
+    e = vdn.Engine(record_pairs=1, **cfg)
+    e.set_mesh(verts, tets)
+    e.set_sites(sites)
+    e.knn_step()
+    e.clip_step()
+    st, vals = e.pairs()
+    mom = e.moments()
+    hist = e.status_hist()
+    e.update_step()
+    new = e.sites()
+    sk, tk = e.site_knn(), e.tet_knn()
+    e.close()
+    return dict(status=st, vals=vals, moments=mom, hist=hist, sites=new, sknn=sk, tknn=tk)
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("form", ["tet", "site"])
+@pytest.mark.parametrize("accum", ["f32", "f64"])
+@pytest.mark.parametrize("moments", ["reference", "robust"])
+def test_pairs_bit_exact(vdn, O, case, form, accum, moments):
+    kind, n, ns = case
+    verts, tets, sites = problem(kind, n, ns)
+    sk, tk = oracle_knn(kind, n, ns, 90, 64)
+    print(f"case={case} form={form} accum={accum} moments={moments}", flush=True)
+    g = _run(vdn, verts, tets, sites, tet_k=64, form=form, accum=accum, moments=moments)
+    np.testing.assert_array_equal(g["sknn"], sk)
+    np.testing.assert_array_equal(g["tknn"], tk)
+    site_mode = O.SITE_F32 if accum == "f32" else O.SITE_FIXED
+    r = O.clip_tets(sites, sk, verts, tets, tk, geom="f32", pair_acc=accum, site_mode=site_mode, moments=moments)
+    bad = np.nonzero(g["status"].astype(np.int32) != r["pair_status"])[0]
+    assert bad.size == 0, f"{bad.size} pair statuses differ, first {bad[:5]}"
+    dv = np.nonzero(np.any(g["vals"] != r["pair_vals"], axis=1))[0]
+    assert dv.size == 0, f"{dv.size} pair values differ, first {dv[:5]}: {g['vals'][dv[:3]]} vs {r['pair_vals'][dv[:3]]}"
+    hist = np.bincount(r["pair_status"], minlength=7)
+    assert [g["hist"][k] for k in vdn.STATUS_NAMES] == hist.tolist()
+    if accum == "f64":
+        np.testing.assert_array_equal(g["moments"], r["site_vals"])
+        np.testing.assert_array_equal(g["sites"], O.lloyd_update(r["site_vals"], sites, fp32=False))
+    else:
+        np.testing.assert_allclose(g["moments"], r["site_vals"], rtol=1e-5, atol=1e-3)
+        np.testing.assert_allclose(g["sites"], O.lloyd_update(r["site_vals"], sites, fp32=True), rtol=1e-5, atol=1e-4)
+
+
+@pytest.mark.parametrize("caps", [(8, 12), (10, 16)])
+def test_overflow_statuses_match(vdn, O, caps):
+    """Tiny caps force vertex/triangle overflows; statuses must agree."""
+    verts, tets, sites = problem("cube", 8, 300)
+    sk, tk = oracle_knn("cube", 8, 300, 90, 64)
+    pcap, tcap = caps
+    for form in ("tet", "site"):
+        g = _run(vdn, verts, tets, sites, tet_k=64, form=form, pcap=pcap, tcap=tcap)
+        r = O.clip_tets(sites, sk, verts, tets, tk, pcap=pcap, tcap=tcap)
+        np.testing.assert_array_equal(g["status"].astype(np.int32), r["pair_status"])
+        np.testing.assert_array_equal(g["vals"], r["pair_vals"])
+        assert (r["pair_status"] <= 1).sum() > 0  # the caps did bite
+
+
+def test_accuracy_vs_fp64_oracle(vdn, O):
+    """FP32 clipping + FP64 accumulation vs the all-FP64 oracle."""
+    verts, tets, sites = problem("cube", 20, 2000)
+    sk, tk = oracle_knn("cube", 20, 2000, 90, 64)
+    g = _run(vdn, verts, tets, sites, tet_k=64, accum="f64", moments="robust")
+    r = O.clip_tets(sites, sk, verts, tets, tk, geom="f64", pair_acc="f64", site_mode=O.SITE_F64,
+                    moments="robust", want_pairs=False)
+    a, b = g["moments"], r["site_vals"]
+    vol_err = np.abs(a[:, 0] - b[:, 0]) / b[:, 0]
+    cen_err = np.linalg.norm(a[:, 1:] / a[:, :1] - b[:, 1:] / b[:, :1], axis=1) / np.linalg.norm(b[:, 1:] / b[:, :1], axis=1)
+    print("volume rel err max/mean", vol_err.max(), vol_err.mean(), "centroid rel err max/mean", cen_err.max(), cen_err.mean())
+    assert vol_err.max() <= 1e-5
+    assert cen_err.max() <= 1e-5
+
+
+def test_single_site_takes_whole_domain(vdn, O):
+    """One site: its cell is the whole mesh (closed form)."""
+    verts, tets, _ = problem("cube", 6, 10)
+    site = np.array([[400.0, 500.0, 600.0]], dtype=np.float32)
+    g = _run(vdn, verts, tets, site, accum="f64", moments="robust")
+    V = verts.astype(np.float64)
+    a, b, c, d = (V[tets[:, j]] for j in range(4))
+    tv = -np.einsum("ij,ij->i", np.cross(b - d, c - d), a - d) / 6
+    cen = ((a + b + c + d) / 4 * tv[:, None]).sum(0) / tv.sum()
+    np.testing.assert_allclose(g["moments"][0, 0], tv.sum(), rtol=1e-6)
+    np.testing.assert_allclose(g["sites"][0], cen, rtol=1e-6)
+
+
+def test_full_size_config2_properties(vdn, O):
+    """998,250 tets / 100k sites / TK=64: invariants that do not need the oracle
+    on all 63.9M pairs, plus an exact check of a sample of pairs."""
+    verts, tets, sites = problem("cube", 55, 100000)
+    e = vdn.Engine(tet_k=64, accum="f64", moments="robust", record_pairs=1)
+    e.set_mesh(verts, tets)
+    e.set_sites(sites)
+    e.knn_step()
+    e.clip_step()
+    st, vals = e.pairs()
+    mom = e.moments()
+    hist = e.status_hist()
+    sk, tk = e.site_knn(), e.tet_knn()
+    e.close()
+    assert sum(hist.values()) == len(tets) * 64
+    V = verts.astype(np.float64)
+    a, b, c, d = (V[tets[:, j]] for j in range(4))
+    tv = -np.einsum("ij,ij->i", np.cross(b - d, c - d), a - d) / 6
+    cover = vals[:, 0].reshape(len(tets), 64).sum(1)
+    rel = np.abs(cover - tv) / tv
+    print("per-tet coverage rel err max", rel.max(), "total", mom[:, 0].sum() / tv.sum() - 1)
+    assert rel.max() < 1e-3
+    assert abs(mom[:, 0].sum() / tv.sum() - 1) < 1e-6
+    # exact check on the pairs of a sample of tets
+    rng = np.random.default_rng(5)
+    tsel = np.sort(rng.choice(len(tets), 4000, replace=False))
+    r = O.clip_tets(sites, sk, verts, tets[tsel], tk[tsel], geom="f32", pair_acc="f64", site_mode=O.SITE_F64,
+                    moments="robust")
+    pidx = (tsel[:, None] * 64 + np.arange(64)[None, :]).ravel()
+    np.testing.assert_array_equal(st[pidx].astype(np.int32), r["pair_status"])
+    np.testing.assert_array_equal(vals[pidx], r["pair_vals"])

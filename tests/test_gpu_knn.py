@@ -1,0 +1,111 @@
+"""Exact kNN on the GPU against the oracle (bit-exact index lists).
+
+The oracle's kNN is itself pinned against brute force in test_oracle.py.
+"""
+import numpy as np
+import pytest
+from conftest import oracle_knn, problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(vdn, verts, tets, sites, **cfg):
+    e = vdn.Engine(**cfg)
+    e.set_mesh(verts, tets)
+    e.set_sites(sites)
+    e.knn_step()
+    return e
+
+
+@pytest.mark.parametrize("case", [("cube", 8, 300), ("cube", 20, 2000), ("cube", 30, 16200), ("torus", 24, 3000)])
+def test_site_and_tet_knn_bit_exact(vdn, case):
+    kind, n, ns = case
+    verts, tets, sites = problem(kind, n, ns)
+    sk, tk = oracle_knn(kind, n, ns, 90, 64)
+    with _engine(vdn, verts, tets, sites, tet_k=64) as e:
+        k, t, _, _ = e.dims()
+        assert (k, t) == (sk.shape[1], tk.shape[1])
+        g_sk, g_tk = e.site_knn(), e.tet_knn()
+    print(f"{case}: site knn mismatched rows {np.sum(np.any(g_sk != sk, axis=1))}, "
+          f"tet knn mismatched rows {np.sum(np.any(g_tk != tk, axis=1))}", flush=True)
+    np.testing.assert_array_equal(g_sk, sk)
+    np.testing.assert_array_equal(g_tk, tk)
+
+
+def test_knn_full_size_config2(vdn, O):
+    """55^3 x 6 tets, 100k sites: the single-solve configuration."""
+    verts, tets, sites = problem("cube", 55, 100000)
+    sk = O.site_knn(sites, 90)
+    with _engine(vdn, verts, tets, sites, tet_k=64) as e:
+        g_sk, g_tk = e.site_knn(), e.tet_knn()
+    np.testing.assert_array_equal(g_sk, sk)
+    # candidate lists: exact on a sample of tets (the oracle's grid kNN on all
+    # 998k tets is checked by the smaller cases)
+    rng = np.random.default_rng(0)
+    idx = rng.choice(len(tets), 20000, replace=False)
+    cent = O.tet_centroids(verts, tets)[idx]
+    ref = O.knn(sites, cent, 64, False)
+    np.testing.assert_array_equal(g_tk[idx], ref)
+
+
+def test_lattice_ties_break_by_index(vdn, O):
+    """Sites on a lattice: many equal distances; ties go to the lower index."""
+    g = np.arange(12, dtype=np.float32) * 40.0 + 50.0
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    sites = np.stack([x.ravel(), y.ravel(), z.ravel()], 1).astype(np.float32)
+    rng = np.random.default_rng(1)
+    sites = sites[rng.permutation(len(sites))]
+    verts, tets, _ = problem("cube", 6, 10)
+    with _engine(vdn, verts, tets, sites, tet_k=27) as e:
+        g_sk, g_tk = e.site_knn(), e.tet_knn()
+    np.testing.assert_array_equal(g_sk, O.knn(sites, sites, 90, True, grid=False))
+    cent = O.tet_centroids(verts, tets)
+    np.testing.assert_array_equal(g_tk, O.knn(sites, cent, 27, False, grid=False))
+
+
+def test_k_larger_than_sites(vdn, O):
+    verts, tets, sites = problem("cube", 4, 5)
+    with _engine(vdn, verts, tets, sites, tet_k=64) as e:
+        k, tk, ns, _ = e.dims()
+        assert k == 4 and tk == 5
+        np.testing.assert_array_equal(e.site_knn(), O.knn(sites, sites, 4, True, grid=False))
+        np.testing.assert_array_equal(e.tet_knn(), O.knn(sites, O.tet_centroids(verts, tets), 5, False, grid=False))
+
+
+def test_knn_dev_reference_layout(vdn, O):
+    """vd_knn_dev: pitched SoA rows in and out, like knn_cuda_global_dev."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(3)
+    ref = rng.random((3000, 3), dtype=np.float32) * 1000
+    qry = rng.random((1000, 3), dtype=np.float32) * 1000
+    k = 17
+    pitch_r, pitch_q = 3072, 1024
+    dref = torch.zeros((3, pitch_r), dtype=torch.float32, device="cuda")
+    dq = torch.zeros((3, pitch_q), dtype=torch.float32, device="cuda")
+    dref[:, :3000] = torch.from_numpy(ref.T.copy()).cuda()
+    dq[:, :1000] = torch.from_numpy(qry.T.copy()).cuda()
+    didx = torch.full((k, pitch_q), -7, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    rc = vdn.lib().vd_knn_dev(dref.data_ptr(), 3000, pitch_r, dq.data_ptr(), 1000, pitch_q, 3, k,
+                              didx.data_ptr(), pitch_q, None)
+    assert rc == 0
+    got = didx.cpu().numpy()[:, :1000].T
+    np.testing.assert_array_equal(got, O.knn(ref, qry, k, False, grid=False))
+    assert vdn.lib().vd_knn_dev(dref.data_ptr(), 3000, pitch_r, dq.data_ptr(), 1000, pitch_q, 2, k,
+                                didx.data_ptr(), pitch_q, None) != 0  # dim != 3 is rejected
+
+
+def test_kn_problem_sorted_ids(vdn, O):
+    """kn_prepare/kn_solve: lists in grid order, entries are grid positions."""
+    _, _, sites = problem("cube", 10, 20000)
+    kn = vdn.KnProblem(sites, 90)
+    kn.solve()
+    perm = kn.permutation().astype(np.int64)
+    assert np.array_equal(np.sort(perm), np.arange(len(sites)))
+    np.testing.assert_array_equal(kn.points(), sites[perm])
+    lists = kn.knearests().astype(np.int64)
+    want = O.site_knn(sites, 90)
+    np.testing.assert_array_equal(perm[lists], want[perm])
+    info = kn.info()
+    assert info.allocated_points == len(sites) and info.dimx >= 1
+    kn.close()
