@@ -15,6 +15,7 @@
 // on the same slot hit distinct banks.
 #include "vd_clip.hpp"
 #include "vd_geom.hpp"
+#include "vd_knn.hpp"
 
 namespace vd {
 
@@ -40,18 +41,58 @@ struct PlaneStoreLocal {
   __device__ __forceinline__ void put(int i, const P4& p) { pl[i * 64 + lane] = make_float4(p.x, p.y, p.z, p.w); }
 };
 
+// Face planes of the lane (ids 0..3) and the wave's bisector table (id 4+v)
+// live in one LDS region; the id picks the address, so a plane fetch is one
+// ds_read_b128 without a branch.
 struct PlaneStoreShared {
   float4* face;        // [4][64] per lane
   const float4* bis;   // [K] shared
   int lane;
-  __device__ __forceinline__ P4 get(int i) const { return i < 4 ? ld4(face, i * 64 + lane) : ld4(bis, i - 4); }
+  __device__ __forceinline__ P4 get(int i) const {
+    const float4* p = i < 4 ? face + (i * 64 + lane) : bis + (i - 4);
+    const float4 v = *p;
+    return {v.x, v.y, v.z, v.w};
+  }
   __device__ __forceinline__ void put(int i, const P4& p) { face[i * 64 + lane] = make_float4(p.x, p.y, p.z, p.w); }
 };
 
-template <class PS>
+// Triangle (= polytope vertex) storage policies.
+// TriPlain: the vertex's plane triple only (the reference's layout); the
+//   security-radius test recomputes every vertex.
+// TriCached: plane triple + the vertex's squared distance to the seed in one
+//   8-byte word.  A vertex is a function of its three planes only, so the
+//   cached value is the one the recomputation would give, bit for bit.
+struct TriPlain {
+  static constexpr bool kCached = false;
+  uint32_t* tr;  // [tcap][64]
+  int lane;
+  __device__ __forceinline__ uint32_t get(int t) const { return tr[t * 64 + lane]; }
+  __device__ __forceinline__ float d2(int) const { return 0.f; }
+  __device__ __forceinline__ void put(int t, uint32_t v, float) { tr[t * 64 + lane] = v; }
+  __device__ __forceinline__ void swap(int a, int b) {
+    const uint32_t x = tr[a * 64 + lane], y = tr[b * 64 + lane];
+    tr[a * 64 + lane] = y;
+    tr[b * 64 + lane] = x;
+  }
+};
+struct TriCached {
+  static constexpr bool kCached = true;
+  uint2* tr;  // [tcap][64]: (plane triple, d2 bits)
+  int lane;
+  __device__ __forceinline__ uint32_t get(int t) const { return tr[t * 64 + lane].x; }
+  __device__ __forceinline__ float d2(int t) const { return __uint_as_float(tr[t * 64 + lane].y); }
+  __device__ __forceinline__ void put(int t, uint32_t v, float d2) { tr[t * 64 + lane] = make_uint2(v, __float_as_uint(d2)); }
+  __device__ __forceinline__ void swap(int a, int b) {
+    const uint2 x = tr[a * 64 + lane], y = tr[b * 64 + lane];
+    tr[a * 64 + lane] = y;
+    tr[b * 64 + lane] = x;
+  }
+};
+
+template <class PS, class TS>
 struct Cell {
   PS ps;
-  uint32_t* tr;  // [tcap][64]
+  TS ts;
   uint8_t* nx;   // [nxcap][64]
   int lane;
   int pcap, tcap;
@@ -60,12 +101,22 @@ struct Cell {
   int status;
   P4 seed;
 
-  __device__ __forceinline__ uint32_t tri(int t) const { return tr[t * 64 + lane]; }
-  __device__ __forceinline__ void set_tri(int t, uint32_t v) { tr[t * 64 + lane] = v; }
+  __device__ __forceinline__ uint32_t tri(int t) const { return ts.get(t); }
   __device__ __forceinline__ int nxt(int p) const { return nx[p * 64 + lane]; }
   __device__ __forceinline__ void set_nxt(int p, int v) { nx[p * 64 + lane] = (uint8_t)v; }
 
   __device__ __forceinline__ P4 vertex(uint32_t t) const { return vertex3(ps.get(tri_a(t)), ps.get(tri_b(t)), ps.get(tri_c(t))); }
+  __device__ __forceinline__ float seed_d2(const P4& X) const {
+    const P4 d = sub3(X, seed);
+    return dot3(d, d);
+  }
+  // Stores triangle t; the cached form also computes its vertex distance.
+  __device__ __forceinline__ void put_tri(int t, uint32_t v) {
+    if constexpr (TS::kCached)
+      ts.put(t, v, seed_d2(vertex(v)));
+    else
+      ts.put(t, v, 0.f);
+  }
 
   // Tet start: planes of faces cbd, acd, bad, abc (voronoi.cu:163-201).
   __device__ void init_tet(const P4& s, const P4 v[4]) {
@@ -76,10 +127,10 @@ struct Cell {
     ps.put(2, tri2plane(v[1], v[0], v[3]));
     ps.put(3, tri2plane(v[0], v[1], v[2]));
     nv = 4;
-    set_tri(0, pack3(1, 3, 2));
-    set_tri(1, pack3(0, 1, 2));
-    set_tri(2, pack3(0, 3, 1));
-    set_tri(3, pack3(0, 2, 3));
+    put_tri(0, pack3(1, 3, 2));
+    put_tri(1, pack3(0, 1, 2));
+    put_tri(2, pack3(0, 3, 1));
+    put_tri(3, pack3(0, 2, 3));
     nt = 4;
     first = kNoPlane;
   }
@@ -89,8 +140,10 @@ struct Cell {
   __device__ bool security_reached(float d2z) const {
     float vmax = 0.f;
     for (int t = 0; t < nt; ++t) {
-      const P4 d = sub3(vertex(tri(t)), seed);
-      vmax = fmaxf(dot3(d, d), vmax);
+      if constexpr (TS::kCached)
+        vmax = fmaxf(ts.d2(t), vmax);
+      else
+        vmax = fmaxf(seed_d2(vertex(tri(t))), vmax);
     }
     return d2z > 4.f * vmax;
   }
@@ -153,10 +206,7 @@ struct Cell {
         if (first == e0) first = nxt(e0);
         set_nxt(e0, kNoPlane);
       }
-      const int last = nt + nr - 1;
-      const uint32_t tl = tri(last);
-      set_tri(last, t);
-      set_tri(cur, tl);
+      ts.swap(cur, nt + nr - 1);
       cur = nt;
       --nr;
     }
@@ -176,8 +226,7 @@ struct Cell {
       const Minors m = vertex_minors(ps.get(tri_a(t)), ps.get(tri_b(t)), ps.get(tri_c(t)));
       if (conflict_det(m, eqn) > 0.f) {
         --nt;
-        set_tri(i, tri(nt));
-        set_tri(nt, t);
+        ts.swap(i, nt);
         ++nr;
       } else {
         ++i;
@@ -210,7 +259,7 @@ struct Cell {
         status = kInconsistentBoundary;
         return;
       }
-      set_tri(nt, pack3(pid, c, n));
+      put_tri(nt, pack3(pid, c, n));
       ++nt;
       c = n;
     } while (c != first);
@@ -316,8 +365,8 @@ __device__ __forceinline__ P4 site_at(const ClipArgs& a, int i) { return ld4(a.s
 // Clips one pair whose cell has been set up; returns through S and status.
 // kernel A's loop (voronoi.cu:679-700) with the bisector of the v-th neighbour
 // under plane id pid(v).
-template <int MF, int ACC, class PS, class PlaneOf>
-__device__ __forceinline__ bool run_pair(const ClipArgs& a, Cell<PS>& c, const int32_t* nbr, PlaneOf plane_of,
+template <int MF, int ACC, class PS, class TS, class PlaneOf>
+__device__ __forceinline__ bool run_pair(const ClipArgs& a, Cell<PS, TS>& c, const int32_t* nbr, PlaneOf plane_of,
                                          PairSums<ACC>& S) {
   const int K = a.K;
   bool stopped = false;
@@ -368,16 +417,21 @@ template <int MF, int ACC>
 __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
   extern __shared__ __align__(16) uint8_t smem[];
   const int lane = threadIdx.x;
-  Cell<PlaneStoreLocal> c;
+  Cell<PlaneStoreLocal, TriPlain> c;
   c.ps.pl = (float4*)smem;
   c.ps.lane = lane;
-  c.tr = (uint32_t*)(c.ps.pl + (size_t)a.pcap * 64);
-  c.nx = (uint8_t*)(c.tr + (size_t)a.tcap * 64);
+  c.ts.tr = (uint32_t*)(c.ps.pl + (size_t)a.pcap * 64);
+  c.ts.lane = lane;
+  c.nx = (uint8_t*)(c.ts.tr + (size_t)a.tcap * 64);
   c.lane = lane;
   c.pcap = a.pcap;
   c.tcap = a.tcap;
-  const long long pair = (long long)blockIdx.x * 64 + lane;
-  const bool live = pair < a.n_pairs;
+  // Either all pairs (one per lane) or the pairs of a retry list (grid-strided).
+  const long long n_items = a.pair_list ? (long long)min(*a.pair_list_count, (unsigned)a.n_pairs) : a.n_pairs;
+  for (long long base = (long long)blockIdx.x * 64; base < n_items; base += (long long)gridDim.x * 64) {
+  const long long item = base + lane;
+  const bool live = item < n_items;
+  const long long pair = !live ? 0 : (a.pair_list ? (long long)a.pair_list[item] : item);
   int status = kNoIntersection;
   PairSums<ACC> S;
   bool acc = false;
@@ -414,6 +468,7 @@ __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
     }
   }
   count_status(a, status, live);
+  }
 }
 
 // ----------------------------------------------------------- site-major ----
@@ -430,13 +485,14 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   float4* bis = (float4*)smem;                          // [K]
   float* bd2 = (float*)(bis + K);                       // [K]
   float4* face = (float4*)(bd2 + ((K + 3) & ~3));       // [4][64]
-  uint32_t* tr = (uint32_t*)(face + 4 * 64);            // [tcap][64]
+  uint2* tr = (uint2*)(face + 4 * 64);                  // [tcap][64]
   uint8_t* nx = (uint8_t*)(tr + (size_t)a.tcap * 64);   // [4+K][64]
-  Cell<PlaneStoreShared> c;
+  Cell<PlaneStoreShared, TriCached> c;
   c.ps.face = face;
   c.ps.bis = bis;
   c.ps.lane = lane;
-  c.tr = tr;
+  c.ts.tr = tr;
+  c.ts.lane = lane;
   c.nx = nx;
   c.lane = lane;
   c.pcap = a.pcap;
@@ -476,14 +532,22 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         PairSums<ACC> S;
         const bool acc = run_pair<MF, ACC>(a, c, nbr, plane_of, S);
         status = c.status;
-        if (acc) accumulate<ACC>(a, s, S);
-        if (a.pair_status) a.pair_status[pair] = (uint8_t)status;
-        if (a.pair_vals) {
-          double* pv = a.pair_vals + 4 * pair;
-          pv[0] = acc ? (double)S.v : 0.0;
-          pv[1] = acc ? (double)S.x : 0.0;
-          pv[2] = acc ? (double)S.y : 0.0;
-          pv[3] = acc ? (double)S.z : 0.0;
+        if (status == kTriangleOverflow && a.retry_list && a.tcap < a.tcap_full) {
+          // this launch's vertex cap is below the configured one: the pair is
+          // recomputed from scratch at the configured caps by the retry pass
+          const unsigned slot = atomicAdd(a.retry_count, 1u);
+          if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = (int)pair;
+          status = -1;
+        } else {
+          if (acc) accumulate<ACC>(a, s, S);
+          if (a.pair_status) a.pair_status[pair] = (uint8_t)status;
+          if (a.pair_vals) {
+            double* pv = a.pair_vals + 4 * pair;
+            pv[0] = acc ? (double)S.v : 0.0;
+            pv[1] = acc ? (double)S.x : 0.0;
+            pv[2] = acc ? (double)S.y : 0.0;
+            pv[3] = acc ? (double)S.z : 0.0;
+          }
         }
       }
       count_status(a, status, live);
@@ -491,60 +555,31 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   }
 }
 
-// Pairs grouped by seed: counting sort of the candidate table by site.
-__global__ void k_pairs_count(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count) {
+// Pairs grouped by seed, and within a seed by candidate rank: a counting sort
+// keyed by (site, rank).  Pairs of low rank are the ones that intersect the
+// site's cell, so grouping by rank puts pairs of similar cost in one wave.
+__global__ void k_pairs_count(const int32_t* tet_knn, long long n_pairs, int TK, int n_sites, int* count) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_pairs;
        i += (long long)gridDim.x * blockDim.x) {
     const int s = tet_knn[i];
-    if ((unsigned)s < (unsigned)n_sites) atomicAdd(count + s, 1);
+    if ((unsigned)s < (unsigned)n_sites) atomicAdd(count + (long long)s * TK + (int)(i % TK), 1);
   }
 }
-__global__ void k_pairs_scatter(const int32_t* tet_knn, long long n_pairs, int n_sites, const int* start,
+__global__ void k_pairs_scatter(const int32_t* tet_knn, long long n_pairs, int TK, int n_sites, const int* start,
                                 int* cursor, int* pair_ids) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_pairs;
        i += (long long)gridDim.x * blockDim.x) {
     const int s = tet_knn[i];
-    if ((unsigned)s < (unsigned)n_sites) pair_ids[start[s] + atomicAdd(cursor + s, 1)] = (int)i;
-  }
-}
-// Sorts each site's pair list (one thread per site; lists are short).
-__global__ void k_pairs_sort(int n_sites, const int* start, int* pair_ids) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n_sites) return;
-  const int b = start[s], e = start[s + 1];
-  for (int i = b + 1; i < e; ++i) {
-    const int v = pair_ids[i];
-    int j = i - 1;
-    while (j >= b && pair_ids[j] > v) {
-      pair_ids[j + 1] = pair_ids[j];
-      --j;
+    if ((unsigned)s < (unsigned)n_sites) {
+      const long long key = (long long)s * TK + (int)(i % TK);
+      pair_ids[start[key] + atomicAdd(cursor + key, 1)] = (int)i;
     }
-    pair_ids[j + 1] = v;
   }
 }
-
-// Exclusive scan over n counts, single workgroup (n up to a few million).
-__global__ __launch_bounds__(1024) void k_scan1(const int* count, int n, int* start, int* cursor) {
-  __shared__ int part[1024];
-  const int per = (n + 1023) / 1024;
-  const int b = threadIdx.x * per, e = min(n, b + per);
-  int s = 0;
-  for (int i = b; i < e; ++i) s += count[i];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
-  for (int i = b; i < e; ++i) {
-    start[i] = run;
-    run += count[i];
-    cursor[i] = 0;
-  }
-  if (threadIdx.x == 1023) start[n] = part[1023];
+// site_start[s] = start[s * TK]
+__global__ void k_pairs_site_start(const int* start, int n_sites, int TK, int* site_start) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s <= n_sites) site_start[s] = start[(long long)s * TK];
 }
 
 // Lloyd update (voronoi.cu:728-749): site = moment / volume where volume != 0.
@@ -584,7 +619,7 @@ __global__ void k_moments_out(int n, int acc_mode, const float* acc_f32, const u
 
 size_t clip_plain_lds(int pcap, int tcap) { return (size_t)64 * (16 * pcap + 4 * tcap + pcap); }
 size_t clip_site_lds(int K, int tcap) {
-  return (size_t)16 * K + 4 * (size_t)((K + 3) & ~3) + 16 * 4 * 64 + (size_t)4 * tcap * 64 + (size_t)(4 + K) * 64;
+  return (size_t)16 * K + 4 * (size_t)((K + 3) & ~3) + 16 * 4 * 64 + (size_t)8 * tcap * 64 + (size_t)(4 + K) * 64;
 }
 
 template <class Kern>
@@ -599,7 +634,7 @@ hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, i
   if (a.n_pairs <= 0) return hipSuccess;
   if (form == 0) {
     const size_t lds = clip_plain_lds(a.pcap, a.tcap);
-    const dim3 grid((unsigned)((a.n_pairs + 63) / 64)), block(64);
+    const dim3 grid(a.pair_list ? (unsigned)n_blocks : (unsigned)((a.n_pairs + 63) / 64)), block(64);
     if (moments == 0 && acc_mode == 0) return launch_lds(k_clip_plain<0, 0>, grid, block, lds, st, a);
     if (moments == 0 && acc_mode == 1) return launch_lds(k_clip_plain<0, 1>, grid, block, lds, st, a);
     if (moments == 1 && acc_mode == 0) return launch_lds(k_clip_plain<1, 0>, grid, block, lds, st, a);
@@ -613,13 +648,16 @@ hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, i
   return launch_lds(k_clip_site<1, 1>, grid, block, lds, st, a);
 }
 
-hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count, int* start,
-                         int* cursor, int* pair_ids, hipStream_t st) {
-  hipLaunchKernelGGL(k_pairs_count, dim3(2048), dim3(256), 0, st, tet_knn, n_pairs, n_sites, count);
-  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, st, count, n_sites, start, cursor);
-  hipLaunchKernelGGL(k_pairs_scatter, dim3(2048), dim3(256), 0, st, tet_knn, n_pairs, n_sites, start, cursor,
+hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int TK, int n_sites, int* count, int* start,
+                         int* cursor, int* site_start, int* pair_ids, int* scan_tmp, hipStream_t st) {
+  const int nkeys = n_sites * TK;
+  hipLaunchKernelGGL(k_pairs_count, dim3(4096), dim3(256), 0, st, tet_knn, n_pairs, TK, n_sites, count);
+  hipError_t e = scan_excl(count, nullptr, nkeys, start, cursor, scan_tmp, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_pairs_scatter, dim3(4096), dim3(256), 0, st, tet_knn, n_pairs, TK, n_sites, start, cursor,
                      pair_ids);
-  hipLaunchKernelGGL(k_pairs_sort, dim3((n_sites + 255) / 256), dim3(256), 0, st, n_sites, start, pair_ids);
+  hipLaunchKernelGGL(k_pairs_site_start, dim3((n_sites + 256) / 256), dim3(256), 0, st, start, n_sites, TK,
+                     site_start);
   return hipGetLastError();
 }
 

@@ -18,7 +18,13 @@ struct ClipArgs {
   int n_sites;
   const int* pair_start;     // site-major form: n_sites + 1
   const int* pair_ids;       // site-major form: n_pairs pair indices grouped by site
-  int pcap, tcap;            // polytope caps (planes, vertices)
+  int pcap, tcap;            // polytope caps (planes, vertices) of this launch
+  int tcap_full;             // configured vertex cap (site-major form: pairs that
+                             // overflow a smaller launch cap go to retry_list)
+  int* retry_list;           // optional, n_pairs entries
+  unsigned* retry_count;     // with retry_list
+  const int* pair_list;      // tet-major form: optional list of pairs to process
+  const unsigned* pair_list_count;
   float* acc_f32;            // n_sites x 4 (vol, mx, my, mz), FP32 accumulation
   unsigned long long* acc_fix;  // n_sites x 4, 2^-20 fixed point of FP64 pair sums
   uint8_t* pair_status;      // optional, n_pairs
@@ -31,11 +37,14 @@ size_t clip_site_lds(int K, int tcap);
 
 // form: 0 tet-major, 1 site-major; moments: 0 reference, 1 robust;
 // acc_mode: 0 FP32 atomics, 1 FP64 pair sums + fixed-point site sums.
+// n_blocks: workgroups of the site-major form, and of the tet-major form when
+// it runs over a.pair_list.
 hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, int n_blocks, hipStream_t st);
 
-// Groups the pairs of an n_tets x TK candidate table by site (deterministic).
-hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count, int* start,
-                         int* cursor, int* pair_ids, hipStream_t st);
+// Groups the pairs of an n_tets x TK candidate table by site, and within a site
+// by rank.  count/start/cursor hold n_sites*TK (+1) keys; site_start n_sites+1.
+hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int TK, int n_sites, int* count, int* start,
+                         int* cursor, int* site_start, int* pair_ids, int* scan_tmp, hipStream_t st);
 
 hipError_t lloyd_update(float4* sites, int n, int acc_mode, const float* acc_f32, const unsigned long long* acc_fix,
                         hipStream_t st);

@@ -57,7 +57,7 @@ struct Tmp {
 
 struct GridTmp {
   Tmp<vd::GridShape> shape;
-  Tmp<int> count, start, cursor, cell_of, ids;
+  Tmp<int> count, start, cursor, cell_of, ids, scan_tmp;
   Tmp<float4> sorted;
   Tmp<int4> offsets;
   int n_offsets = 0;
@@ -66,7 +66,8 @@ struct GridTmp {
     max_cells = vd::grid_max_cells(n, density);
     hipError_t e;
     if ((e = shape.alloc(1)) || (e = count.alloc(max_cells)) || (e = start.alloc((size_t)max_cells + 1)) ||
-        (e = cursor.alloc(max_cells)) || (e = cell_of.alloc(n)) || (e = ids.alloc(n)) || (e = sorted.alloc(n)))
+        (e = cursor.alloc(max_cells)) || (e = cell_of.alloc(n)) || (e = ids.alloc(n)) || (e = sorted.alloc(n)) ||
+        (e = scan_tmp.alloc(vd::scan_tmp_ints(max_cells))))
       return e;
     const std::vector<int4> t = vd::knn_offset_table();
     if ((e = offsets.alloc(t.size()))) return e;
@@ -83,6 +84,7 @@ struct GridTmp {
     g.cell_of = cell_of.p;
     g.sorted_ids = ids.p;
     g.sorted_pts = sorted.p;
+    g.scan_tmp = scan_tmp.p;
     return g;
   }
 };

@@ -74,13 +74,13 @@ struct vd_ctx {
   int n_offsets = 0;
   // grid
   DevBuf<vd::GridShape> shape;
-  DevBuf<int> g_count, g_start, g_cursor, g_cell_of, g_sorted_ids;
+  DevBuf<int> g_count, g_start, g_cursor, g_cell_of, g_sorted_ids, g_scan_tmp, p_scan_tmp;
   DevBuf<float4> g_sorted_pts;
   int max_cells = 0;
   // kNN results
   DevBuf<int32_t> site_knn, tet_knn;
   // site-major pairs
-  DevBuf<int> p_count, p_start, p_cursor, p_ids;
+  DevBuf<int> p_count, p_start, p_cursor, p_site_start, p_ids;
   // accumulators
   DevBuf<float> acc_f32;
   DevBuf<unsigned long long> acc_fix;
@@ -88,6 +88,8 @@ struct vd_ctx {
   DevBuf<uint8_t> pair_status;
   DevBuf<double> pair_vals;
   DevBuf<unsigned long long> hist;
+  DevBuf<int> retry_list;
+  DevBuf<unsigned> retry_count;
 
   // HIP events of the phases of the last kRing iterations: slot = iteration % kRing;
   // [0] start, [1] after kNN, [2] after pair grouping, [3] after clip, [4] after update
@@ -131,10 +133,14 @@ int check_config(vd_ctx* ctx, const vd_config& c) {
   if (c.form != VD_FORM_TET_MAJOR && c.form != VD_FORM_SITE_MAJOR) return ctx->fail(VD_ERR_ARG, "bad form %d", c.form);
   if (!(c.grid_density > 0.05f && c.grid_density < 1000.f))
     return ctx->fail(VD_ERR_ARG, "grid_density out of range");
-  if (c.form == VD_FORM_TET_MAJOR && vd::clip_plain_lds(c.pcap, c.tcap) > 160 * 1024)
-    return ctx->fail(VD_ERR_ARG, "pcap/tcap too large for the tet-major form's LDS");
-  if (c.form == VD_FORM_SITE_MAJOR && vd::clip_site_lds(c.site_k, c.tcap) > 160 * 1024)
-    return ctx->fail(VD_ERR_ARG, "site_k/tcap too large for the site-major form's LDS");
+  // the tet-major kernel also serves the site-major form's overflow retries
+  if (vd::clip_plain_lds(c.pcap, c.tcap) > 160 * 1024)
+    return ctx->fail(VD_ERR_ARG, "pcap/tcap too large for the tet-major kernel's LDS");
+  const int fast_tcap = c.site_tcap > 0 ? std::min(c.site_tcap, c.tcap) : std::min(c.tcap, 32);
+  if (c.form == VD_FORM_SITE_MAJOR && vd::clip_site_lds(c.site_k, fast_tcap) > 160 * 1024)
+    return ctx->fail(VD_ERR_ARG, "site_k/site_tcap too large for the site-major form's LDS");
+  if (c.site_tcap != 0 && (c.site_tcap < 8 || c.site_tcap > 1024))
+    return ctx->fail(VD_ERR_ARG, "site_tcap must be 0 or in [8,1024], got %d", c.site_tcap);
   if (c.form == VD_FORM_SITE_MAJOR && c.site_k + 4 > 254)
     return ctx->fail(VD_ERR_ARG, "site-major form needs site_k <= 250");
   return VD_OK;
@@ -149,6 +155,7 @@ int ensure_grid(vd_ctx* ctx, int n) {
   VD_HIP(ctx->g_cell_of.ensure(n), "alloc grid");
   VD_HIP(ctx->g_sorted_ids.ensure(n), "alloc grid");
   VD_HIP(ctx->g_sorted_pts.ensure(n), "alloc grid");
+  VD_HIP(ctx->g_scan_tmp.ensure(vd::scan_tmp_ints(mc)), "alloc grid");
   ctx->max_cells = mc;
   return VD_OK;
 }
@@ -163,6 +170,7 @@ vd::GridBufs grid_bufs(vd_ctx* ctx) {
   gb.cell_of = ctx->g_cell_of.p;
   gb.sorted_ids = ctx->g_sorted_ids.p;
   gb.sorted_pts = ctx->g_sorted_pts.p;
+  gb.scan_tmp = ctx->g_scan_tmp.p;
   return gb;
 }
 
@@ -201,6 +209,7 @@ void vd_config_default(vd_config* c) {
   c->grid_density = 3.1f;
   c->record_pairs = 0;
   c->clip_blocks = 0;
+  c->site_tcap = 0;
 }
 
 int vd_ctx_create(vd_ctx** out, int device, const vd_config* cfg) {
@@ -247,10 +256,13 @@ void vd_ctx_destroy(vd_ctx* ctx) {
   ctx->verts.release(); ctx->sites.release(); ctx->tets.release(); ctx->offsets.release();
   ctx->shape.release(); ctx->g_count.release(); ctx->g_start.release(); ctx->g_cursor.release();
   ctx->g_cell_of.release(); ctx->g_sorted_ids.release(); ctx->g_sorted_pts.release();
+  ctx->g_scan_tmp.release(); ctx->p_scan_tmp.release();
   ctx->site_knn.release(); ctx->tet_knn.release();
   ctx->p_count.release(); ctx->p_start.release(); ctx->p_cursor.release(); ctx->p_ids.release();
+  ctx->p_site_start.release();
   ctx->acc_f32.release(); ctx->acc_fix.release(); ctx->moments.release();
   ctx->pair_status.release(); ctx->pair_vals.release(); ctx->hist.release();
+  ctx->retry_list.release(); ctx->retry_count.release();
   for (auto& row : ctx->ev)
     for (auto& e : row)
       if (e) (void)hipEventDestroy(e);
@@ -408,24 +420,47 @@ int vd_clip_step(vd_ctx* ctx) {
   a.status_hist = ctx->hist.p;
   int blocks = 0;
   if (c.form == VD_FORM_SITE_MAJOR) {
-    VD_HIP(ctx->p_count.ensure(n), "alloc pairs");
-    VD_HIP(ctx->p_start.ensure((size_t)n + 1), "alloc pairs");
-    VD_HIP(ctx->p_cursor.ensure(n), "alloc pairs");
+    const size_t nkeys = (size_t)n * std::max(ctx->TK, 1);
+    VD_HIP(ctx->p_count.ensure(nkeys), "alloc pairs");
+    VD_HIP(ctx->p_start.ensure(nkeys + 1), "alloc pairs");
+    VD_HIP(ctx->p_cursor.ensure(nkeys), "alloc pairs");
+    VD_HIP(ctx->p_site_start.ensure((size_t)n + 1), "alloc pairs");
     VD_HIP(ctx->p_ids.ensure(std::max(n_pairs, 1LL)), "alloc pairs");
-    VD_HIP(vd::fill_u32(ctx->p_count.p, n, 0u, ctx->stream), "clear pairs");
-    VD_HIP(vd::pairs_by_site(ctx->tet_knn.p, n_pairs, n, ctx->p_count.p, ctx->p_start.p, ctx->p_cursor.p,
-                             ctx->p_ids.p, ctx->stream),
+    VD_HIP(ctx->p_scan_tmp.ensure(vd::scan_tmp_ints((int)nkeys)), "alloc pairs");
+    VD_HIP(vd::fill_u32(ctx->p_count.p, (long long)nkeys, 0u, ctx->stream), "clear pairs");
+    VD_HIP(vd::pairs_by_site(ctx->tet_knn.p, n_pairs, ctx->TK, n, ctx->p_count.p, ctx->p_start.p, ctx->p_cursor.p,
+                             ctx->p_site_start.p, ctx->p_ids.p, ctx->p_scan_tmp.p, ctx->stream),
            "pairs by site");
     VD_HIP(hipEventRecord(ctx->E(2), ctx->stream), "event");
     ctx->ev_pairs = true;
-    a.pair_start = ctx->p_start.p;
+    a.pair_start = ctx->p_site_start.p;
     a.pair_ids = ctx->p_ids.p;
     blocks = c.clip_blocks > 0 ? c.clip_blocks : n;
+    const int fast_tcap = c.site_tcap > 0 ? std::min(c.site_tcap, c.tcap) : std::min(c.tcap, 32);
+    if (fast_tcap < c.tcap) {
+      VD_HIP(ctx->retry_list.ensure(std::max(n_pairs, 1LL)), "alloc retry list");
+      VD_HIP(ctx->retry_count.ensure(1), "alloc retry list");
+      VD_HIP(vd::fill_u32(ctx->retry_count.p, 1, 0u, ctx->stream), "clear retry count");
+      a.retry_list = ctx->retry_list.p;
+      a.retry_count = ctx->retry_count.p;
+    }
+    a.tcap = fast_tcap;
+    a.tcap_full = c.tcap;
   } else {
     VD_HIP(hipEventRecord(ctx->E(2), ctx->stream), "event");
     ctx->ev_pairs = true;
   }
   VD_HIP(vd::clip_launch(a, c.form, c.moments, c.accum, blocks, ctx->stream), "clip");
+  if (a.retry_list) {
+    // pairs that overflowed the site-major launch's vertex cap, at the configured caps
+    vd::ClipArgs r = a;
+    r.tcap = c.tcap;
+    r.pair_list = a.retry_list;
+    r.pair_list_count = a.retry_count;
+    r.retry_list = nullptr;
+    r.retry_count = nullptr;
+    VD_HIP(vd::clip_launch(r, VD_FORM_TET_MAJOR, c.moments, c.accum, 256, ctx->stream), "clip retry");
+  }
   VD_HIP(hipEventRecord(ctx->E(3), ctx->stream), "event");
   ctx->ev_clip = true;
   ctx->have_clip = true;

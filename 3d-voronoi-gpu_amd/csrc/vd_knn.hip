@@ -155,33 +155,6 @@ __global__ void k_grid_count(const float4* pts, int n, const GridShape* gs, int*
   atomicAdd(count + c, 1);
 }
 
-// Exclusive scan of count[0..ncell) into start[0..ncell], one workgroup;
-// also clears the scatter cursors.
-__global__ __launch_bounds__(kScanThreads) void k_grid_scan(const GridShape* gs, const int* count, int* start,
-                                                             int* cursor) {
-  __shared__ int part[kScanThreads];
-  const int ncell = gs->ncell;
-  const int per = (ncell + kScanThreads - 1) / kScanThreads;
-  const int b = threadIdx.x * per, e = min(ncell, b + per);
-  int s = 0;
-  for (int i = b; i < e; ++i) s += count[i];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = 1; off < kScanThreads; off <<= 1) {
-    const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  int run = threadIdx.x ? part[threadIdx.x - 1] : 0;
-  for (int i = b; i < e; ++i) {
-    start[i] = run;
-    run += count[i];
-    cursor[i] = 0;
-  }
-  if (threadIdx.x == kScanThreads - 1) start[ncell] = part[kScanThreads - 1];
-}
-
 __global__ void k_grid_scatter(int n, const int* cell_of, const int* start, int* cursor, int* sorted_ids) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -466,6 +439,105 @@ hipError_t fill_u32(void* p, long long n, uint32_t v, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Exclusive scan, three passes: per-block scans of kScanTile items, a scan of
+// the block totals, then the block offsets added back.  Also clears `cursor`.
+constexpr int kScanItems = 8;
+constexpr int kScanTile = 1024 * kScanItems;
+
+__device__ __forceinline__ int scan_n(const int* n_dev, int n_max) { return n_dev ? min(*n_dev, n_max) : n_max; }
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(const int* in, const int* n_dev, int n_max, int* out,
+                                                      int* cursor, int* tile_sum) {
+  __shared__ int part[1024];
+  const int n = scan_n(n_dev, n_max);
+  const long long base = (long long)blockIdx.x * kScanTile;
+  if (base >= n && blockIdx.x > 0) return;
+  int v[kScanItems];
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const long long i = base + (long long)j * 1024 + threadIdx.x;
+    v[j] = i < n ? in[i] : 0;
+  }
+  // thread t owns items base + t*kScanItems .. ; reorder through LDS for that
+  __shared__ int tile[kScanTile];
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) tile[j * 1024 + threadIdx.x] = v[j];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    v[j] = tile[threadIdx.x * kScanItems + j];
+    s += v[j];
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int x = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += x;
+    __syncthreads();
+  }
+  int run = part[threadIdx.x] - s;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    tile[threadIdx.x * kScanItems + j] = run;
+    run += v[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const long long i = base + (long long)j * 1024 + threadIdx.x;
+    if (i < n) {
+      out[i] = tile[j * 1024 + threadIdx.x];
+      if (cursor) cursor[i] = 0;
+    }
+  }
+  if (threadIdx.x == 1023) tile_sum[blockIdx.x] = part[1023];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(const int* n_dev, int n_max, int* tile_sum) {
+  __shared__ int part[1024];
+  const int n = scan_n(n_dev, n_max);
+  const int tiles = (n + kScanTile - 1) / kScanTile;
+  const int per = (tiles + 1023) / 1024;
+  const int b = threadIdx.x * per, e = min(tiles, b + per);
+  int s = 0;
+  for (int i = b; i < e; ++i) s += tile_sum[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int x = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += x;
+    __syncthreads();
+  }
+  int run = part[threadIdx.x] - s;
+  for (int i = b; i < e; ++i) {
+    const int t = tile_sum[i];
+    tile_sum[i] = run;
+    run += t;
+  }
+  if (threadIdx.x == 1023) tile_sum[tiles] = part[1023];
+}
+
+__global__ void k_scan_add(const int* n_dev, int n_max, int* out, const int* tile_sum) {
+  const int n = scan_n(n_dev, n_max);
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += tile_sum[i / kScanTile];
+  if (i == n) out[n] = tile_sum[(n + kScanTile - 1) / kScanTile];
+}
+
+int scan_tmp_ints(int n_max) { return (n_max + kScanTile - 1) / kScanTile + 2; }
+
+hipError_t scan_excl(const int* in, const int* n_dev, int n_max, int* out, int* cursor, int* tmp, hipStream_t st) {
+  const int tiles = std::max(1, (n_max + kScanTile - 1) / kScanTile);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(1024), 0, st, in, n_dev, n_max, out, cursor, tmp);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, n_dev, n_max, tmp);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)(((long long)n_max + 1 + 255) / 256)), dim3(256), 0, st, n_dev,
+                     n_max, out, tmp);
+  return hipGetLastError();
+}
+
 std::vector<int4> knn_offset_table() {
   std::vector<int4> t;
   const int R = kOffsetRadius;
@@ -499,7 +571,8 @@ hipError_t grid_build(const GridBufs& gb, const float4* pts, int n, const float*
   if (e != hipSuccess) return e;
   const int nb = (n + 255) / 256;
   if (n > 0) hipLaunchKernelGGL(k_grid_count, dim3(nb), dim3(256), 0, st, pts, n, gb.shape, gb.cell_of, gb.count);
-  hipLaunchKernelGGL(k_grid_scan, dim3(1), dim3(kScanThreads), 0, st, gb.shape, gb.count, gb.start, gb.cursor);
+  e = scan_excl(gb.count, &gb.shape->ncell, gb.max_cells, gb.start, gb.cursor, gb.scan_tmp, st);
+  if (e != hipSuccess) return e;
   if (n > 0)
     hipLaunchKernelGGL(k_grid_scatter, dim3(nb), dim3(256), 0, st, n, gb.cell_of, gb.start, gb.cursor,
                        gb.sorted_ids);

@@ -29,7 +29,14 @@ struct GridBufs {
   int* cell_of;       // n
   int* sorted_ids;    // n
   float4* sorted_pts; // n
+  int* scan_tmp;      // scan_tmp_ints(max_cells)
 };
+
+// Exclusive scan of in[0..n) into out[0..n] (out[n] = total), n = *n_dev when
+// n_dev is given (capped at n_max), else n_max; clears cursor[0..n) if given.
+// tmp holds scan_tmp_ints(n_max) ints.
+int scan_tmp_ints(int n_max);
+hipError_t scan_excl(const int* in, const int* n_dev, int n_max, int* out, int* cursor, int* tmp, hipStream_t st);
 
 std::vector<int4> knn_offset_table();
 int grid_max_cells(int n, float density);

@@ -36,6 +36,7 @@ class Config(ctypes.Structure):
         ("grid_density", ctypes.c_float),
         ("record_pairs", ctypes.c_int),
         ("clip_blocks", ctypes.c_int),
+        ("site_tcap", ctypes.c_int),
     ]
 
 

@@ -65,6 +65,9 @@ typedef struct vd_config {
   float grid_density; /* sites per kNN grid cell */
   int record_pairs;   /* keep per-pair status and values for vd_get_pairs */
   int clip_blocks;    /* workgroups of the site-major launch; 0 = automatic */
+  int site_tcap;      /* vertex cap of the site-major launch; pairs that exceed it are
+                         recomputed at tcap, so results do not depend on it;
+                         0 = automatic (min(tcap, 32)) */
 } vd_config;
 
 typedef struct vd_timing {
@@ -79,7 +82,7 @@ typedef struct vd_ctx vd_ctx;
 
 /* Defaults: site_k 90, tet_k 0, pcap 64, tcap 96, VD_ACCUM_F32,
  * VD_MOMENTS_REFERENCE, VD_FORM_SITE_MAJOR, grid_density 3.1
- * (knearests.cu:243), record_pairs 0, clip_blocks 0. */
+ * (knearests.cu:243), record_pairs 0, clip_blocks 0, site_tcap 0. */
 void vd_config_default(vd_config* cfg);
 
 /* Context on HIP device `device`.  Replaces the device selection and buffer

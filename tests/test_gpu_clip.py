@@ -73,6 +73,22 @@ def test_overflow_statuses_match(vdn, O, caps):
         assert (r["pair_status"] <= 1).sum() > 0  # the caps did bite
 
 
+@pytest.mark.parametrize("site_tcap", [8, 12])
+def test_site_form_overflow_retry(vdn, O, site_tcap):
+    """The site-major launch with a tiny vertex cap: pairs that overflow it are
+    recomputed at the configured caps, so every result equals the single-pass
+    tet-major form at the configured caps."""
+    verts, tets, sites = problem("cube", 20, 2000)
+    sk, tk = oracle_knn("cube", 20, 2000, 90, 64)
+    g = _run(vdn, verts, tets, sites, tet_k=64, form="site", accum="f64", site_tcap=site_tcap)
+    r = O.clip_tets(sites, sk, verts, tets, tk, geom="f32", pair_acc="f64", site_mode=O.SITE_FIXED)
+    np.testing.assert_array_equal(g["status"].astype(np.int32), r["pair_status"])
+    np.testing.assert_array_equal(g["vals"], r["pair_vals"])
+    np.testing.assert_array_equal(g["moments"], r["site_vals"])
+    hist = np.bincount(r["pair_status"], minlength=7)
+    assert [g["hist"][k] for k in vdn.STATUS_NAMES] == hist.tolist()
+
+
 def test_accuracy_vs_fp64_oracle(vdn, O):
     """FP32 clipping + FP64 accumulation vs the all-FP64 oracle."""
     verts, tets, sites = problem("cube", 20, 2000)
@@ -121,9 +137,15 @@ def test_full_size_config2_properties(vdn, O):
     tv = -np.einsum("ij,ij->i", np.cross(b - d, c - d), a - d) / 6
     cover = vals[:, 0].reshape(len(tets), 64).sum(1)
     rel = np.abs(cover - tv) / tv
-    print("per-tet coverage rel err max", rel.max(), "total", mom[:, 0].sum() / tv.sum() - 1)
-    assert rel.max() < 1e-3
-    assert abs(mom[:, 0].sum() / tv.sum() - 1) < 1e-6
+    # A pair flagged security_radius_not_reached was clipped by the site's K=90
+    # neighbours only; its cell can reach farther than the true cell (the
+    # reference accumulates it anyway), so coverage is exact only elsewhere.
+    srnr = (st.reshape(len(tets), 64) == 3).any(1)
+    print("per-tet coverage rel err max", rel[~srnr].max(), "tets with srnr pairs", srnr.sum(),
+          "total", mom[:, 0].sum() / tv.sum() - 1)
+    assert srnr.sum() < 1e-3 * len(tets)
+    assert rel[~srnr].max() < 1e-3
+    assert abs(mom[:, 0].sum() / tv.sum() - 1) < 1e-4
     # exact check on the pairs of a sample of tets
     rng = np.random.default_rng(5)
     tsel = np.sort(rng.choice(len(tets), 4000, replace=False))

@@ -72,27 +72,58 @@ def test_k_larger_than_sites(vdn, O):
         np.testing.assert_array_equal(e.tet_knn(), O.knn(sites, O.tet_centroids(verts, tets), 5, False, grid=False))
 
 
+class _Hip:
+    """Device buffers through the HIP runtime the library itself links (torch
+    ships a different HIP runtime and cannot share the process)."""
+
+    def __init__(self):
+        import ctypes
+
+        self.ct = ctypes
+        self.L = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+        self.L.hipMalloc.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+        self.L.hipFree.argtypes = [ctypes.c_void_p]
+        self.L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.L.hipDeviceSynchronize.argtypes = []
+
+    def to_dev(self, a):
+        p = self.ct.c_void_p()
+        assert self.L.hipMalloc(self.ct.byref(p), a.nbytes) == 0
+        assert self.L.hipMemcpy(p, a.ctypes.data_as(self.ct.c_void_p), a.nbytes, 1) == 0  # host to device
+        return p
+
+    def to_host(self, p, a):
+        assert self.L.hipDeviceSynchronize() == 0
+        assert self.L.hipMemcpy(a.ctypes.data_as(self.ct.c_void_p), p, a.nbytes, 2) == 0  # device to host
+        return a
+
+    def free(self, p):
+        self.L.hipFree(p)
+
+
 def test_knn_dev_reference_layout(vdn, O):
     """vd_knn_dev: pitched SoA rows in and out, like knn_cuda_global_dev."""
-    torch = pytest.importorskip("torch")
     rng = np.random.default_rng(3)
     ref = rng.random((3000, 3), dtype=np.float32) * 1000
     qry = rng.random((1000, 3), dtype=np.float32) * 1000
     k = 17
     pitch_r, pitch_q = 3072, 1024
-    dref = torch.zeros((3, pitch_r), dtype=torch.float32, device="cuda")
-    dq = torch.zeros((3, pitch_q), dtype=torch.float32, device="cuda")
-    dref[:, :3000] = torch.from_numpy(ref.T.copy()).cuda()
-    dq[:, :1000] = torch.from_numpy(qry.T.copy()).cuda()
-    didx = torch.full((k, pitch_q), -7, dtype=torch.int32, device="cuda")
-    torch.cuda.synchronize()
-    rc = vdn.lib().vd_knn_dev(dref.data_ptr(), 3000, pitch_r, dq.data_ptr(), 1000, pitch_q, 3, k,
-                              didx.data_ptr(), pitch_q, None)
-    assert rc == 0
-    got = didx.cpu().numpy()[:, :1000].T
-    np.testing.assert_array_equal(got, O.knn(ref, qry, k, False, grid=False))
-    assert vdn.lib().vd_knn_dev(dref.data_ptr(), 3000, pitch_r, dq.data_ptr(), 1000, pitch_q, 2, k,
-                                didx.data_ptr(), pitch_q, None) != 0  # dim != 3 is rejected
+    h_ref = np.zeros((3, pitch_r), dtype=np.float32)
+    h_ref[:, :3000] = ref.T
+    h_q = np.zeros((3, pitch_q), dtype=np.float32)
+    h_q[:, :1000] = qry.T
+    h_idx = np.full((k, pitch_q), -7, dtype=np.int32)
+    hip = _Hip()
+    dref, dq, didx = hip.to_dev(h_ref), hip.to_dev(h_q), hip.to_dev(h_idx)
+    try:
+        rc = vdn.lib().vd_knn_dev(dref, 3000, pitch_r, dq, 1000, pitch_q, 3, k, didx, pitch_q, None)
+        assert rc == 0
+        got = hip.to_host(didx, np.empty_like(h_idx))[:, :1000].T
+        np.testing.assert_array_equal(got, O.knn(ref, qry, k, False, grid=False))
+        assert vdn.lib().vd_knn_dev(dref, 3000, pitch_r, dq, 1000, pitch_q, 2, k, didx, pitch_q, None) != 0
+    finally:
+        for p in (dref, dq, didx):
+            hip.free(p)
 
 
 def test_kn_problem_sorted_ids(vdn, O):
