@@ -21,6 +21,12 @@ namespace vd {
 
 namespace {
 
+// Emptiness test of the site-major form: required depth of every tet vertex
+// beyond a bisector (distance, frame units) and the relative rounding bound of
+// the FP32 plane evaluation fma(nz,z,fma(ny,y,nx*x)) + w.
+constexpr float kCullMargin = 1e-2f;
+constexpr float kCullRound = 1.2e-6f;
+
 __device__ __forceinline__ uint32_t pack3(int a, int b, int c) { return (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16); }
 __device__ __forceinline__ int tri_a(uint32_t t) { return (int)(t & 255u); }
 __device__ __forceinline__ int tri_b(uint32_t t) { return (int)((t >> 8) & 255u); }
@@ -404,6 +410,12 @@ __device__ __forceinline__ bool run_pair(const ClipArgs& a, Cell<PS, TS>& c, con
     c.template moments_ref<ACC>(S);
   else
     c.template moments_robust<ACC>(S);
+  // vertices formed by parallel planes (degenerate input) give non-finite sums;
+  // such a pair is not accumulated
+  if (!(isfinite((double)S.v) && isfinite((double)S.x) && isfinite((double)S.y) && isfinite((double)S.z))) {
+    c.status = kNeedsExactPredicates;
+    return false;
+  }
   if (fabs((double)S.v) < 0.1) {
     c.status = kNoIntersection;
     return false;
@@ -497,6 +509,9 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   c.lane = lane;
   c.pcap = a.pcap;
   c.tcap = a.tcap;
+  float4* cull = (float4*)(nx + (size_t)(4 + K) * 64);  // [K] (offset is a multiple of 16)
+  int* surv = (int*)(cull + K);                                                             // [128]
+  const int depth = min(a.cull_depth, K);
   for (int s = blockIdx.x; s < a.n_sites; s += gridDim.x) {
     const int pb = a.pair_start[s], pe = a.pair_start[s + 1];
     if (pb == pe) continue;
@@ -511,14 +526,16 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
       bis[v] = make_float4(eq.x, eq.y, eq.z, eq.w);
       const P4 d = sub3(zp, sp);
       bd2[v] = dot3(d, d);
+      // emptiness test terms: |n|_1, |w|, and the required depth beyond the
+      // plane (kCullMargin in distance, times |n|)
+      cull[v] = make_float4(fabsf(eq.x) + fabsf(eq.y) + fabsf(eq.z), fabsf(eq.w), kCullMargin * sqrtf(bd2[v]), 0.f);
     }
     __syncthreads();
-    for (int base = pb; base < pe; base += 64) {
-      const int i = base + lane;
-      const bool live = i < pe;
+
+    // Full clipping of one batch: lane takes `pair` when `live`.
+    auto clip_batch = [&](long long pair, bool live) {
       int status = kNoIntersection;
       if (live) {
-        const long long pair = a.pair_ids[i];
         const int t = (int)(pair / a.TK);
         const int4 tv = a.tets[t];
         const P4 v[4] = {ld4(a.verts, tv.x), ld4(a.verts, tv.y), ld4(a.verts, tv.z), ld4(a.verts, tv.w)};
@@ -551,6 +568,73 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         }
       }
       count_status(a, status, live);
+    };
+
+    // Pass over the site's pairs: pairs whose tet lies entirely beyond one of
+    // the first `depth` bisectors (every vertex deeper than the rounding
+    // bound of the plane evaluation plus kCullMargin) have an empty
+    // intersection with the cell and end here as no_intersection; the others
+    // are queued and clipped 64 at a time.
+    int queued = 0;
+    for (int base = pb; base < pe; base += 64) {
+      const int i = base + lane;
+      const bool live = i < pe;
+      const long long pair = live ? a.pair_ids[i] : 0;
+      bool culled = false;
+      if (depth > 0) {
+        P4 v[4] = {};
+        float pmax = 0.f;
+        if (live) {
+          const int4 tv = a.tets[(int)(pair / a.TK)];
+          v[0] = ld4(a.verts, tv.x);
+          v[1] = ld4(a.verts, tv.y);
+          v[2] = ld4(a.verts, tv.z);
+          v[3] = ld4(a.verts, tv.w);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) pmax = fmaxf(pmax, fmaxf(fabsf(v[j].x), fmaxf(fabsf(v[j].y), fabsf(v[j].z))));
+        }
+        bool open = live;
+        for (int q = 0; q < depth; ++q) {
+          if (__ballot(open) == 0) break;
+          if (open) {
+            const float4 e = bis[q];
+            const float4 m = cull[q];
+            const float lim = -fmaf(kCullRound, fmaf(m.x, pmax, m.y), m.z);
+            bool all = true;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) all = all && (fmaf(e.z, v[j].z, fmaf(e.y, v[j].y, e.x * v[j].x)) + e.w < lim);
+            if (all) {
+              culled = true;
+              open = false;
+            }
+          }
+        }
+        if (culled && a.pair_status) a.pair_status[pair] = (uint8_t)kNoIntersection;
+        if (culled && a.pair_vals) {
+          double* pv = a.pair_vals + 4 * pair;
+          pv[0] = pv[1] = pv[2] = pv[3] = 0.0;
+        }
+        count_status(a, kNoIntersection, culled);
+      }
+      const bool keep = live && !culled;
+      const unsigned long long mask = __ballot(keep);
+      const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+      if (keep) surv[queued + below] = (int)pair;
+      queued += __popcll(mask);
+      __syncthreads();
+      if (queued >= 64) {
+        const int mine = surv[lane];
+        const int rest = lane + 64 < queued ? surv[lane + 64] : 0;
+        __syncthreads();
+        if (lane + 64 < queued) surv[lane] = rest;
+        queued -= 64;
+        __syncthreads();
+        clip_batch(mine, true);
+      }
+    }
+    if (queued > 0) {
+      const int mine = lane < queued ? surv[lane] : 0;
+      clip_batch(mine, lane < queued);
     }
   }
 }
@@ -619,7 +703,8 @@ __global__ void k_moments_out(int n, int acc_mode, const float* acc_f32, const u
 
 size_t clip_plain_lds(int pcap, int tcap) { return (size_t)64 * (16 * pcap + 4 * tcap + pcap); }
 size_t clip_site_lds(int K, int tcap) {
-  return (size_t)16 * K + 4 * (size_t)((K + 3) & ~3) + 16 * 4 * 64 + (size_t)8 * tcap * 64 + (size_t)(4 + K) * 64;
+  return (size_t)16 * K + 4 * (size_t)((K + 3) & ~3) + 16 * 4 * 64 + (size_t)8 * tcap * 64 + (size_t)(4 + K) * 64 +
+         (size_t)16 * K + 4 * 128;
 }
 
 template <class Kern>

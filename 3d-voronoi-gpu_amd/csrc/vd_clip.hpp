@@ -23,6 +23,7 @@ struct ClipArgs {
                              // overflow a smaller launch cap go to retry_list)
   int* retry_list;           // optional, n_pairs entries
   unsigned* retry_count;     // with retry_list
+  int cull_depth;            // site-major form: bisectors tried by the emptiness test (0 = off)
   const int* pair_list;      // tet-major form: optional list of pairs to process
   const unsigned* pair_list_count;
   float* acc_f32;            // n_sites x 4 (vol, mx, my, mz), FP32 accumulation

@@ -139,6 +139,8 @@ int check_config(vd_ctx* ctx, const vd_config& c) {
   const int fast_tcap = c.site_tcap > 0 ? std::min(c.site_tcap, c.tcap) : std::min(c.tcap, 32);
   if (c.form == VD_FORM_SITE_MAJOR && vd::clip_site_lds(c.site_k, fast_tcap) > 160 * 1024)
     return ctx->fail(VD_ERR_ARG, "site_k/site_tcap too large for the site-major form's LDS");
+  if (c.cull_depth < 0 || c.cull_depth > 256)
+    return ctx->fail(VD_ERR_ARG, "cull_depth must be in [0,256], got %d", c.cull_depth);
   if (c.site_tcap != 0 && (c.site_tcap < 8 || c.site_tcap > 1024))
     return ctx->fail(VD_ERR_ARG, "site_tcap must be 0 or in [8,1024], got %d", c.site_tcap);
   if (c.form == VD_FORM_SITE_MAJOR && c.site_k + 4 > 254)
@@ -210,6 +212,7 @@ void vd_config_default(vd_config* c) {
   c->record_pairs = 0;
   c->clip_blocks = 0;
   c->site_tcap = 0;
+  c->cull_depth = 32;
 }
 
 int vd_ctx_create(vd_ctx** out, int device, const vd_config* cfg) {
@@ -446,6 +449,7 @@ int vd_clip_step(vd_ctx* ctx) {
     }
     a.tcap = fast_tcap;
     a.tcap_full = c.tcap;
+    a.cull_depth = c.cull_depth;
   } else {
     VD_HIP(hipEventRecord(ctx->E(2), ctx->stream), "event");
     ctx->ev_pairs = true;

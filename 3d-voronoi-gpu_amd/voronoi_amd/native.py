@@ -37,6 +37,7 @@ class Config(ctypes.Structure):
         ("record_pairs", ctypes.c_int),
         ("clip_blocks", ctypes.c_int),
         ("site_tcap", ctypes.c_int),
+        ("cull_depth", ctypes.c_int),
     ]
 
 

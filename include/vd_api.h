@@ -68,6 +68,10 @@ typedef struct vd_config {
   int site_tcap;      /* vertex cap of the site-major launch; pairs that exceed it are
                          recomputed at tcap, so results do not depend on it;
                          0 = automatic (min(tcap, 32)) */
+  int cull_depth;     /* site-major form: a pair whose tet lies entirely beyond one of the
+                         seed's first cull_depth bisectors (by more than the FP32 rounding
+                         bound plus 0.01) has an empty intersection and is not clipped;
+                         0 = off */
 } vd_config;
 
 typedef struct vd_timing {
@@ -82,7 +86,7 @@ typedef struct vd_ctx vd_ctx;
 
 /* Defaults: site_k 90, tet_k 0, pcap 64, tcap 96, VD_ACCUM_F32,
  * VD_MOMENTS_REFERENCE, VD_FORM_SITE_MAJOR, grid_density 3.1
- * (knearests.cu:243), record_pairs 0, clip_blocks 0, site_tcap 0. */
+ * (knearests.cu:243), record_pairs 0, clip_blocks 0, site_tcap 0, cull_depth 32. */
 void vd_config_default(vd_config* cfg);
 
 /* Context on HIP device `device`.  Replaces the device selection and buffer

@@ -513,6 +513,27 @@ struct PairOut {
   int32_t accumulated;
 };
 
+// Moments of a finished pair: dropped below 0.1 (no_intersection,
+// voronoi.cu:599-603); a non-finite result (vertices formed by parallel planes
+// in a degenerate configuration) is not accumulated and reported as
+// needs_exact_predicates -- the reference would add the NaN.
+template <class R, class A>
+inline void finish_pair(Poly<R>& c, int mform, PairOut& o) {
+  A vol, mx, my, mz;
+  const bool keep = moments<R, A>(c, mform, vol, mx, my, mz);
+  if (!(std::isfinite((double)vol) && std::isfinite((double)mx) && std::isfinite((double)my) &&
+        std::isfinite((double)mz))) {
+    c.status = needs_exact_predicates;
+    return;
+  }
+  if (!keep) {
+    c.status = no_intersection;
+    return;
+  }
+  o.vol = (double)vol; o.mx = (double)mx; o.my = (double)my; o.mz = (double)mz;
+  o.accumulated = 1;
+}
+
 // Kernel A for one (tet, seed) pair (voronoi.cu:662-701).  `nbr` holds the
 // seed's k nearest other sites, nearest first.
 template <class R, class A>
@@ -533,15 +554,7 @@ inline PairOut clip_pair_tet(Poly<R>& c, const float* sites, const int32_t* nbr,
     }
   }
   if (k == 0 || !c.security_reached(site(nbr[k - 1]))) c.status = security_radius_not_reached;
-  if (c.status != no_intersection) {
-    A vol, mx, my, mz;
-    if (moments<R, A>(c, mform, vol, mx, my, mz)) {
-      o.vol = (double)vol; o.mx = (double)mx; o.my = (double)my; o.mz = (double)mz;
-      o.accumulated = 1;
-    } else {
-      c.status = no_intersection;
-    }
-  }
+  if (c.status != no_intersection) finish_pair<R, A>(c, mform, o);
   o.status = c.status;
   return o;
 }
@@ -573,15 +586,7 @@ inline PairOut clip_pair_cell(Poly<R>& c, const V4<R> tv[4], int mform) {
     c.clip(tri2plane(tv[face[i][0]], tv[face[i][1]], tv[face[i][2]]), true);
     if (c.status == no_intersection) break;
   }
-  if (c.status != no_intersection) {
-    A vol, mx, my, mz;
-    if (moments<R, A>(c, mform, vol, mx, my, mz)) {
-      o.vol = (double)vol; o.mx = (double)mx; o.my = (double)my; o.mz = (double)mz;
-      o.accumulated = 1;
-    } else {
-      c.status = no_intersection;
-    }
-  }
+  if (c.status != no_intersection) finish_pair<R, A>(c, mform, o);
   o.status = c.status;
   return o;
 }

@@ -61,16 +61,26 @@ def test_pairs_bit_exact(vdn, O, case, form, accum, moments):
 
 @pytest.mark.parametrize("caps", [(8, 12), (10, 16)])
 def test_overflow_statuses_match(vdn, O, caps):
-    """Tiny caps force vertex/triangle overflows; statuses must agree."""
+    """Tiny caps force vertex/triangle overflows; statuses must agree.
+
+    With the site-major form's emptiness test on, a pair whose tet lies beyond
+    one bisector is reported as no_intersection even where the reference, at
+    such tiny caps, overflows first; neither accumulates anything, so only
+    those labels may differ."""
     verts, tets, sites = problem("cube", 8, 300)
     sk, tk = oracle_knn("cube", 8, 300, 90, 64)
     pcap, tcap = caps
-    for form in ("tet", "site"):
-        g = _run(vdn, verts, tets, sites, tet_k=64, form=form, pcap=pcap, tcap=tcap)
-        r = O.clip_tets(sites, sk, verts, tets, tk, pcap=pcap, tcap=tcap)
-        np.testing.assert_array_equal(g["status"].astype(np.int32), r["pair_status"])
+    r = O.clip_tets(sites, sk, verts, tets, tk, pcap=pcap, tcap=tcap)
+    assert (r["pair_status"] <= 1).sum() > 0  # the caps did bite
+    for form, cull in (("tet", 0), ("site", 0), ("site", 32)):
+        g = _run(vdn, verts, tets, sites, tet_k=64, form=form, pcap=pcap, tcap=tcap, cull_depth=cull)
         np.testing.assert_array_equal(g["vals"], r["pair_vals"])
-        assert (r["pair_status"] <= 1).sum() > 0  # the caps did bite
+        st = g["status"].astype(np.int32)
+        if cull == 0:
+            np.testing.assert_array_equal(st, r["pair_status"])
+        else:
+            diff = st != r["pair_status"]
+            assert np.all(st[diff] == 6) and np.all(r["pair_status"][diff] <= 2)
 
 
 @pytest.mark.parametrize("site_tcap", [8, 12])
