@@ -218,24 +218,62 @@ struct Cell {
     }
   }
 
-  // Clip by plane eqn stored under id `pid` (voronoi.cu:443-486).
-  __device__ void clip(int pid, const P4& eqn) {
+  // Clip by plane eqn stored under id `pid` (voronoi.cu:443-486); the plane is
+  // the bisector of the seed with a site at squared distance d2z.
+  __device__ void clip(int pid, const P4& eqn, float d2z) {
     if (nv >= pcap) {
       status = kVertexOverflow;
       return;
     }
     ++nv;
     nr = 0;
-    int i = 0;
-    while (i < nt) {
-      const uint32_t t = tri(i);
-      const Minors m = vertex_minors(ps.get(tri_a(t)), ps.get(tri_b(t)), ps.get(tri_c(t)));
-      if (conflict_det(m, eqn) > 0.f) {
-        --nt;
-        ts.swap(i, nt);
-        ++nr;
-      } else {
-        ++i;
+    // With cached vertex distances: a vertex within 0.9 * |z - s| / 2 of the
+    // seed is closer to s than to z by at least 0.05 |z - s|, far beyond the
+    // FP32 error of the determinant, so the test's outcome (no conflict) is
+    // known without computing it.
+    if constexpr (TS::kCached) {
+      // (1) candidates: vertices outside that ball (one bit per slot, tcap <= 32);
+      // (2) the determinant only for them, so a wave pays for the largest
+      //     candidate count rather than the largest vertex count;
+      // (3) the reference's swap loop replayed from the flags: a vertex's
+      //     flag depends on its planes only, so the resulting order is the
+      //     one the reference's in-loop tests produce.
+      const float near2 = 0.2025f * d2z;
+      uint32_t cand = 0;
+      for (int t = 0; t < nt; ++t)
+        if (!(ts.d2(t) < near2)) cand |= 1u << t;
+      uint32_t conf = 0;
+      while (cand) {
+        const int t = __builtin_ctz(cand);
+        cand &= cand - 1;
+        const uint32_t tt = tri(t);
+        const Minors m = vertex_minors(ps.get(tri_a(tt)), ps.get(tri_b(tt)), ps.get(tri_c(tt)));
+        if (conflict_det(m, eqn) > 0.f) conf |= 1u << t;
+      }
+      int i = 0;
+      while (i < nt) {
+        if ((conf >> i) & 1u) {
+          --nt;
+          ts.swap(i, nt);
+          const uint32_t bn = (conf >> nt) & 1u;
+          conf = (conf & ~((1u << i) | (1u << nt))) | (bn << i);
+          ++nr;
+        } else {
+          ++i;
+        }
+      }
+    } else {
+      int i = 0;
+      while (i < nt) {
+        const uint32_t t = tri(i);
+        const Minors m = vertex_minors(ps.get(tri_a(t)), ps.get(tri_b(t)), ps.get(tri_c(t)));
+        if (conflict_det(m, eqn) > 0.f) {
+          --nt;
+          ts.swap(i, nt);
+          ++nr;
+        } else {
+          ++i;
+        }
       }
     }
     if (nr == 0) {
@@ -386,7 +424,7 @@ __device__ __forceinline__ bool run_pair(const ClipArgs& a, Cell<PS, TS>& c, con
     P4 eq;
     float d2z;
     plane_of(v, z, pid, eq, d2z);
-    c.clip(pid, eq);
+    c.clip(pid, eq, d2z);
     if (c.security_reached(d2z)) {
       stopped = true;
       break;
@@ -639,31 +677,106 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   }
 }
 
-// Pairs grouped by seed, and within a seed by candidate rank: a counting sort
-// keyed by (site, rank).  Pairs of low rank are the ones that intersect the
-// site's cell, so grouping by rank puts pairs of similar cost in one wave.
-__global__ void k_pairs_count(const int32_t* tet_knn, long long n_pairs, int TK, int n_sites, int* count) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_pairs;
-       i += (long long)gridDim.x * blockDim.x) {
-    const int s = tet_knn[i];
-    if ((unsigned)s < (unsigned)n_sites) atomicAdd(count + (long long)s * TK + (int)(i % TK), 1);
+// Pairs grouped by seed: a counting sort over sites.  A block takes a run of
+// kGroupPairs consecutive pairs (neighbouring tets share most candidates) and
+// counts them per site in an LDS hash table, then adds each site's count to
+// the global counter once; the scatter pass reserves one range per (block,
+// site) the same way.  Sites that do not fit the table go through global
+// atomics directly.  The order inside a site's list is unspecified.
+constexpr int kGroupPairs = 8192;
+constexpr int kHashBits = 12;
+constexpr int kHashSlots = 1 << kHashBits;
+constexpr int kEmptySlot = -1;
+
+__device__ __forceinline__ unsigned hash_slot(int site) { return ((unsigned)site * 2654435761u) >> (32 - kHashBits); }
+
+__device__ int hash_insert(int* keys, int site) {
+  unsigned h = hash_slot(site);
+  for (int probe = 0; probe < kHashSlots; ++probe) {
+    const int k = keys[h];
+    if (k == site) return (int)h;
+    if (k == kEmptySlot) {
+      const int prev = atomicCAS(&keys[h], kEmptySlot, site);
+      if (prev == kEmptySlot || prev == site) return (int)h;
+    }
+    h = (h + 1) & (kHashSlots - 1);
   }
+  return -1;
 }
-__global__ void k_pairs_scatter(const int32_t* tet_knn, long long n_pairs, int TK, int n_sites, const int* start,
-                                int* cursor, int* pair_ids) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_pairs;
-       i += (long long)gridDim.x * blockDim.x) {
-    const int s = tet_knn[i];
-    if ((unsigned)s < (unsigned)n_sites) {
-      const long long key = (long long)s * TK + (int)(i % TK);
-      pair_ids[start[key] + atomicAdd(cursor + key, 1)] = (int)i;
+
+__device__ int hash_find(const int* keys, int site) {
+  unsigned h = hash_slot(site);
+  for (int probe = 0; probe < kHashSlots; ++probe) {
+    const int k = keys[h];
+    if (k == site) return (int)h;
+    if (k == kEmptySlot) return -1;
+    h = (h + 1) & (kHashSlots - 1);
+  }
+  return -1;
+}
+
+__global__ __launch_bounds__(256) void k_pairs_count(const int32_t* tet_knn, long long n_pairs, int n_sites,
+                                                      int* count) {
+  __shared__ int keys[kHashSlots];
+  __shared__ int vals[kHashSlots];
+  for (int i = threadIdx.x; i < kHashSlots; i += blockDim.x) {
+    keys[i] = kEmptySlot;
+    vals[i] = 0;
+  }
+  __syncthreads();
+  const long long base = (long long)blockIdx.x * kGroupPairs;
+  for (int j = threadIdx.x; j < kGroupPairs; j += blockDim.x) {
+    const long long p = base + j;
+    if (p >= n_pairs) break;
+    const int s = tet_knn[p];
+    if ((unsigned)s >= (unsigned)n_sites) continue;
+    const int h = hash_insert(keys, s);
+    if (h >= 0)
+      atomicAdd(&vals[h], 1);
+    else
+      atomicAdd(count + s, 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHashSlots; i += blockDim.x)
+    if (keys[i] != kEmptySlot) atomicAdd(count + keys[i], vals[i]);
+}
+
+__global__ __launch_bounds__(256) void k_pairs_scatter(const int32_t* tet_knn, long long n_pairs, int n_sites,
+                                                        const int* start, int* cursor, int* pair_ids) {
+  __shared__ int keys[kHashSlots];
+  __shared__ int vals[kHashSlots];
+  __shared__ int offs[kHashSlots];
+  for (int i = threadIdx.x; i < kHashSlots; i += blockDim.x) {
+    keys[i] = kEmptySlot;
+    vals[i] = 0;
+  }
+  __syncthreads();
+  const long long base = (long long)blockIdx.x * kGroupPairs;
+  for (int j = threadIdx.x; j < kGroupPairs; j += blockDim.x) {
+    const long long p = base + j;
+    if (p >= n_pairs) break;
+    const int s = tet_knn[p];
+    if ((unsigned)s >= (unsigned)n_sites) continue;
+    const int h = hash_insert(keys, s);
+    if (h >= 0) atomicAdd(&vals[h], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHashSlots; i += blockDim.x) {
+    if (keys[i] != kEmptySlot) {
+      offs[i] = start[keys[i]] + atomicAdd(cursor + keys[i], vals[i]);
+      vals[i] = 0;
     }
   }
-}
-// site_start[s] = start[s * TK]
-__global__ void k_pairs_site_start(const int* start, int n_sites, int TK, int* site_start) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s <= n_sites) site_start[s] = start[(long long)s * TK];
+  __syncthreads();
+  for (int j = threadIdx.x; j < kGroupPairs; j += blockDim.x) {
+    const long long p = base + j;
+    if (p >= n_pairs) break;
+    const int s = tet_knn[p];
+    if ((unsigned)s >= (unsigned)n_sites) continue;
+    const int h = hash_find(keys, s);
+    const int pos = h >= 0 ? offs[h] + atomicAdd(&vals[h], 1) : start[s] + atomicAdd(cursor + s, 1);
+    pair_ids[pos] = (int)p;
+  }
 }
 
 // Lloyd update (voronoi.cu:728-749): site = moment / volume where volume != 0.
@@ -733,16 +846,15 @@ hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, i
   return launch_lds(k_clip_site<1, 1>, grid, block, lds, st, a);
 }
 
-hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int TK, int n_sites, int* count, int* start,
-                         int* cursor, int* site_start, int* pair_ids, int* scan_tmp, hipStream_t st) {
-  const int nkeys = n_sites * TK;
-  hipLaunchKernelGGL(k_pairs_count, dim3(4096), dim3(256), 0, st, tet_knn, n_pairs, TK, n_sites, count);
-  hipError_t e = scan_excl(count, nullptr, nkeys, start, cursor, scan_tmp, st);
+hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count, int* start, int* cursor,
+                         int* pair_ids, int* scan_tmp, hipStream_t st) {
+  const unsigned blocks = (unsigned)((n_pairs + kGroupPairs - 1) / kGroupPairs);
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pairs_count, dim3(blocks), dim3(256), 0, st, tet_knn, n_pairs, n_sites, count);
+  hipError_t e = scan_excl(count, nullptr, n_sites, start, cursor, scan_tmp, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_pairs_scatter, dim3(4096), dim3(256), 0, st, tet_knn, n_pairs, TK, n_sites, start, cursor,
+  hipLaunchKernelGGL(k_pairs_scatter, dim3(blocks), dim3(256), 0, st, tet_knn, n_pairs, n_sites, start, cursor,
                      pair_ids);
-  hipLaunchKernelGGL(k_pairs_site_start, dim3((n_sites + 256) / 256), dim3(256), 0, st, start, n_sites, TK,
-                     site_start);
   return hipGetLastError();
 }
 

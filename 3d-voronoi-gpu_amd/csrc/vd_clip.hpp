@@ -42,10 +42,11 @@ size_t clip_site_lds(int K, int tcap);
 // it runs over a.pair_list.
 hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, int n_blocks, hipStream_t st);
 
-// Groups the pairs of an n_tets x TK candidate table by site, and within a site
-// by rank.  count/start/cursor hold n_sites*TK (+1) keys; site_start n_sites+1.
-hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int TK, int n_sites, int* count, int* start,
-                         int* cursor, int* site_start, int* pair_ids, int* scan_tmp, hipStream_t st);
+// Groups the pairs of an n_tets x TK candidate table by site: site s owns
+// pair_ids[start[s] .. start[s+1]).  count must be zero; count/cursor hold
+// n_sites ints, start n_sites + 1, scan_tmp scan_tmp_ints(n_sites).
+hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count, int* start, int* cursor,
+                         int* pair_ids, int* scan_tmp, hipStream_t st);
 
 hipError_t lloyd_update(float4* sites, int n, int acc_mode, const float* acc_f32, const unsigned long long* acc_fix,
                         hipStream_t st);

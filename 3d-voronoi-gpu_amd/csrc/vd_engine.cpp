@@ -141,8 +141,8 @@ int check_config(vd_ctx* ctx, const vd_config& c) {
     return ctx->fail(VD_ERR_ARG, "site_k/site_tcap too large for the site-major form's LDS");
   if (c.cull_depth < 0 || c.cull_depth > 256)
     return ctx->fail(VD_ERR_ARG, "cull_depth must be in [0,256], got %d", c.cull_depth);
-  if (c.site_tcap != 0 && (c.site_tcap < 8 || c.site_tcap > 1024))
-    return ctx->fail(VD_ERR_ARG, "site_tcap must be 0 or in [8,1024], got %d", c.site_tcap);
+  if (c.site_tcap != 0 && (c.site_tcap < 8 || c.site_tcap > 32))
+    return ctx->fail(VD_ERR_ARG, "site_tcap must be 0 or in [8,32], got %d", c.site_tcap);
   if (c.form == VD_FORM_SITE_MAJOR && c.site_k + 4 > 254)
     return ctx->fail(VD_ERR_ARG, "site-major form needs site_k <= 250");
   return VD_OK;
@@ -423,20 +423,18 @@ int vd_clip_step(vd_ctx* ctx) {
   a.status_hist = ctx->hist.p;
   int blocks = 0;
   if (c.form == VD_FORM_SITE_MAJOR) {
-    const size_t nkeys = (size_t)n * std::max(ctx->TK, 1);
-    VD_HIP(ctx->p_count.ensure(nkeys), "alloc pairs");
-    VD_HIP(ctx->p_start.ensure(nkeys + 1), "alloc pairs");
-    VD_HIP(ctx->p_cursor.ensure(nkeys), "alloc pairs");
-    VD_HIP(ctx->p_site_start.ensure((size_t)n + 1), "alloc pairs");
+    VD_HIP(ctx->p_count.ensure(n), "alloc pairs");
+    VD_HIP(ctx->p_start.ensure((size_t)n + 1), "alloc pairs");
+    VD_HIP(ctx->p_cursor.ensure(n), "alloc pairs");
     VD_HIP(ctx->p_ids.ensure(std::max(n_pairs, 1LL)), "alloc pairs");
-    VD_HIP(ctx->p_scan_tmp.ensure(vd::scan_tmp_ints((int)nkeys)), "alloc pairs");
-    VD_HIP(vd::fill_u32(ctx->p_count.p, (long long)nkeys, 0u, ctx->stream), "clear pairs");
-    VD_HIP(vd::pairs_by_site(ctx->tet_knn.p, n_pairs, ctx->TK, n, ctx->p_count.p, ctx->p_start.p, ctx->p_cursor.p,
-                             ctx->p_site_start.p, ctx->p_ids.p, ctx->p_scan_tmp.p, ctx->stream),
+    VD_HIP(ctx->p_scan_tmp.ensure(vd::scan_tmp_ints(n)), "alloc pairs");
+    VD_HIP(vd::fill_u32(ctx->p_count.p, n, 0u, ctx->stream), "clear pairs");
+    VD_HIP(vd::pairs_by_site(ctx->tet_knn.p, n_pairs, n, ctx->p_count.p, ctx->p_start.p, ctx->p_cursor.p, ctx->p_ids.p,
+                             ctx->p_scan_tmp.p, ctx->stream),
            "pairs by site");
     VD_HIP(hipEventRecord(ctx->E(2), ctx->stream), "event");
     ctx->ev_pairs = true;
-    a.pair_start = ctx->p_site_start.p;
+    a.pair_start = ctx->p_start.p;
     a.pair_ids = ctx->p_ids.p;
     blocks = c.clip_blocks > 0 ? c.clip_blocks : n;
     const int fast_tcap = c.site_tcap > 0 ? std::min(c.site_tcap, c.tcap) : std::min(c.tcap, 32);

@@ -107,6 +107,29 @@ def cpu_baseline(verts, tets, sites, budget_s):
     )
 
 
+def device_copy(mb):
+    """Device-to-device copy of `mb` MiB through the HIP runtime the engine links:
+    measured HBM bandwidth, and the calibration dispatch for PMC byte counters."""
+    import ctypes
+
+    L = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+    n = mb << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert L.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert L.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    L.hipMemset(a, 1, ctypes.c_size_t(n))
+    L.hipDeviceSynchronize()
+    best = 1e9
+    for _ in range(5):
+        t0 = time.perf_counter()
+        assert L.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0  # device to device
+        L.hipDeviceSynchronize()
+        best = min(best, time.perf_counter() - t0)
+    L.hipFree(a)
+    L.hipFree(b)
+    return 2 * n / best / 1e9
+
+
 def flops_per_pair(ev):
     f = (W_PAIR * ev["pairs"] + W_CLIP * ev["clips"] + W_CONFLICT * ev["conflict_tests"]
          + W_SECURE_VTX * ev["secure_evals"] + W_SECURE_CHECK * ev["secure_checks"] + W_ACC_VTX * ev["acc_tris"])
@@ -121,6 +144,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU work for the baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0, help="also time a device copy of this size (MiB)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -139,9 +163,15 @@ def main():
     log(f"fast: {ms_step:.2f} ms/iter wall; per-iter device: knn {ts['knn_ms']/n:.2f} pairs {ts['pairs_ms']/n:.2f} "
         f"clip {ts['clip_ms']/n:.2f} update {ts['update_ms']/n:.3f} ms; status {hist}")
 
-    pwall, pts, _ = run_engine(native, verts, tets, sites, args.plain_steps, 1, form="tet", accum="f32")
-    pn = pts["iters"]
-    log(f"plain: {1e3 * pwall / args.plain_steps:.2f} ms/iter wall; clip {pts['clip_ms']/pn:.2f} ms")
+    plain = None
+    if args.plain_steps > 0:
+        pwall, pts, _ = run_engine(native, verts, tets, sites, args.plain_steps, 1, form="tet", accum="f32")
+        pn = pts["iters"]
+        plain = (args.plain_steps / pwall, pts["clip_ms"] / pn)
+        log(f"plain: {1e3 * pwall / args.plain_steps:.2f} ms/iter wall; clip {pts['clip_ms']/pn:.2f} ms")
+    copy_gbs = device_copy(args.copy_mb) if args.copy_mb > 0 else None
+    if copy_gbs:
+        log(f"device copy of {args.copy_mb} MiB: {copy_gbs:.0f} GB/s (read + write)")
 
     n_pairs = len(tets) * TK
     cpu = None
@@ -178,9 +208,10 @@ def main():
                    "clip_precision": "fp32", "accumulation": "fp64 (exact fixed-point site sums)",
                    "launch": "site-major"},
         "phase_ms": {k: round(ts[k] / n, 3) for k in ("knn_ms", "pairs_ms", "clip_ms", "update_ms", "total_ms")},
-        "plain_fp32_iters_per_s": round(args.plain_steps / pwall, 4),
-        "plain_clip_ms": round(pts["clip_ms"] / pn, 3),
-        "speedup_vs_plain": round(value / (args.plain_steps / pwall), 3),
+        "plain_fp32_iters_per_s": None if plain is None else round(plain[0], 4),
+        "plain_clip_ms": None if plain is None else round(plain[1], 3),
+        "speedup_vs_plain": None if plain is None else round(value / plain[0], 3),
+        "device_copy_gbs": None if copy_gbs is None else round(copy_gbs, 1),
         "roofline": roof,
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["iters_per_s"], 5), "unit": "iters/s", "cores": cpu["cores"], "kind": "port",

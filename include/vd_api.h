@@ -65,9 +65,9 @@ typedef struct vd_config {
   float grid_density; /* sites per kNN grid cell */
   int record_pairs;   /* keep per-pair status and values for vd_get_pairs */
   int clip_blocks;    /* workgroups of the site-major launch; 0 = automatic */
-  int site_tcap;      /* vertex cap of the site-major launch; pairs that exceed it are
-                         recomputed at tcap, so results do not depend on it;
-                         0 = automatic (min(tcap, 32)) */
+  int site_tcap;      /* vertex cap of the site-major launch, at most 32; pairs that
+                         exceed it are recomputed at tcap, so results do not depend
+                         on it; 0 = automatic (min(tcap, 32)) */
   int cull_depth;     /* site-major form: a pair whose tet lies entirely beyond one of the
                          seed's first cull_depth bisectors (by more than the FP32 rounding
                          bound plus 0.01) has an empty intersection and is not clipped;

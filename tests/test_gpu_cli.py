@@ -18,8 +18,11 @@ def test_cli_on_the_reference_fixture(vdn, tmp_path):
     from voronoi_amd import fileio, meshgen
 
     exe = os.path.join(BUILD, "VolumeVoronoiGPU")
+    # the FP64 form: its site sums do not depend on the order of the atomics,
+    # so the two runs below agree to the 6 printed digits
+    env = dict(os.environ, VD_ACCUM="f64")
     r = subprocess.run([exe, os.path.join(GOLDEN, "joint.tet"), os.path.join(GOLDEN, "joint.xyz"), "2", "0"],
-                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120, env=env)
     print(r.stdout[-400:], r.stderr[-400:])
     assert r.returncode == 0
     for f in ("out.xyz", "iter_0.xyz", "iter_1.xyz", "record.csv"):
@@ -32,13 +35,13 @@ def test_cli_on_the_reference_fixture(vdn, tmp_path):
     # the same two iterations through the C ABI
     verts, tets = fileio.read_tet(os.path.join(GOLDEN, "joint.tet"))
     sites = fileio.read_xyz(os.path.join(GOLDEN, "joint.xyz"))
-    with vdn.Engine() as e:
+    with vdn.Engine(accum="f64") as e:
         e.set_mesh(meshgen.normalize(verts), tets)
         e.set_sites(sites)
         e.lloyd(2)
         want = e.sites()
     got = fileio.read_xyz(str(tmp_path / "out.xyz"))
-    np.testing.assert_allclose(got, want, rtol=1e-5)  # out.xyz holds 6 significant digits
+    np.testing.assert_allclose(got, want, rtol=6e-6)  # out.xyz holds 6 significant digits
 
 
 def test_knearests_and_knncuda_shims(O, tmp_path):
