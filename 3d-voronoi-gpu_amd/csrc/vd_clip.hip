@@ -13,6 +13,8 @@
 //
 // Polytope state lives in LDS, slot-major ([slot][lane]) so that lanes working
 // on the same slot hit distinct banks.
+#include <algorithm>
+
 #include "vd_clip.hpp"
 #include "vd_geom.hpp"
 #include "vd_knn.hpp"
@@ -524,156 +526,294 @@ __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
 // ----------------------------------------------------------- site-major ----
 //
 // Pairs grouped by seed: site s owns pair_ids[pair_start[s] .. pair_start[s+1]).
-// One wavefront per site (grid-strided over sites); lanes take the site's pairs
-// 64 at a time.
+// A wavefront takes sites one at a time from a global counter.  For each site
+// it builds the bisector table once (shared by the wave), runs the emptiness
+// test over the site's pairs 64 at a time and queues the survivors in an LDS
+// ring.  Every lane clips its own pair one bisector per loop trip and takes
+// the next queued pair as soon as it is done, so lanes do not wait for the
+// slowest pair of a group.  Two table slots let lanes start on the next site
+// while others finish the previous one.  Tet corners come from a per-tet
+// record (3 float4 per tet), one dependent load per pair.  A pair sees the
+// same planes in the same order as in kernel A, so its result is the same bit
+// for bit.
+
+constexpr int kQueue = 128;           // ring of queued (pair, slot) entries
+constexpr float kInvalidNbr = -1.f;   // table mark of an out-of-range neighbour id
+// Lane states.  A lane whose clipping is done waits until kMomentsGroup lanes
+// (or at least as many as still clip) are done too: the volume pass then runs
+// for all of them at once instead of once per finishing lane.
+constexpr int kIdle = 0, kClipping = 1, kWaiting = 2;
+constexpr int kMomentsGroup = 24;
+
+// Corners of tet t from its record (x0 y0 z0 x1 | y1 z1 x2 y2 | z2 x3 y3 z3).
+__device__ __forceinline__ void tet_corners(const ClipArgs& a, int t, P4 v[4]) {
+  const float4* r = a.tet_xyz + 3 * (size_t)t;
+  const float4 q0 = r[0], q1 = r[1], q2 = r[2];
+  v[0] = {q0.x, q0.y, q0.z, 0.f};
+  v[1] = {q0.w, q1.x, q1.y, 0.f};
+  v[2] = {q1.z, q1.w, q2.x, 0.f};
+  v[3] = {q2.y, q2.z, q2.w, 0.f};
+}
+
+__host__ __device__ inline size_t site_lds_layout(int K, int tcap, size_t* off) {
+  const size_t K4 = (size_t)((K + 3) & ~3);
+  size_t o = 0;
+  off[0] = o; o += 16 * 2 * K4;        // bis   float4 [2][K4]
+  off[1] = o; o += 8 * 2 * K4;         // cterm float2 [2][K4]
+  off[2] = o; o += 4 * 2 * K4;         // bd2   float  [2][K4]
+  off[3] = o; o += 16 * 4 * 64;        // face  float4 [4][64]
+  off[4] = o; o += 8 * (size_t)tcap * 64;  // tr uint2 [tcap][64]
+  off[5] = o; o += 8 * kQueue;         // queue int2 [kQueue]
+  off[6] = o; o += 4 * 8;              // hist  unsigned [8]
+  off[7] = o; o += (size_t)(4 + K) * 64;  // nx u8 [4+K][64]
+  return (o + 15) & ~(size_t)15;
+}
 
 template <int MF, int ACC>
 __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   extern __shared__ __align__(16) uint8_t smem[];
   const int lane = threadIdx.x;
   const int K = a.K;
-  float4* bis = (float4*)smem;                          // [K]
-  float* bd2 = (float*)(bis + K);                       // [K]
-  float4* face = (float4*)(bd2 + ((K + 3) & ~3));       // [4][64]
-  uint2* tr = (uint2*)(face + 4 * 64);                  // [tcap][64]
-  uint8_t* nx = (uint8_t*)(tr + (size_t)a.tcap * 64);   // [4+K][64]
+  const int K4 = (K + 3) & ~3;
+  size_t off[8];
+  site_lds_layout(K, a.tcap, off);
+  float4* bis = (float4*)(smem + off[0]);
+  float2* cterm = (float2*)(smem + off[1]);
+  float* bd2 = (float*)(smem + off[2]);
+  int2* queue = (int2*)(smem + off[5]);
+  unsigned* hist = (unsigned*)(smem + off[6]);
   Cell<PlaneStoreShared, TriCached> c;
-  c.ps.face = face;
+  c.ps.face = (float4*)(smem + off[3]);
   c.ps.bis = bis;
   c.ps.lane = lane;
-  c.ts.tr = tr;
+  c.ts.tr = (uint2*)(smem + off[4]);
   c.ts.lane = lane;
-  c.nx = nx;
+  c.nx = smem + off[7];
   c.lane = lane;
   c.pcap = a.pcap;
   c.tcap = a.tcap;
-  float4* cull = (float4*)(nx + (size_t)(4 + K) * 64);  // [K] (offset is a multiple of 16)
-  int* surv = (int*)(cull + K);                                                             // [128]
+  if (lane < 8) hist[lane] = 0;
   const int depth = min(a.cull_depth, K);
-  for (int s = blockIdx.x; s < a.n_sites; s += gridDim.x) {
-    const int pb = a.pair_start[s], pe = a.pair_start[s + 1];
-    if (pb == pe) continue;
-    const P4 sp = site_at(a, s);
-    const int32_t* nbr = a.site_knn + (size_t)s * K;
-    __syncthreads();
-    for (int v = lane; v < K; v += 64) {
-      const int z = nbr[v];
-      if ((unsigned)z >= (unsigned)a.n_sites) continue;  // run_pair rejects such a neighbour
-      const P4 zp = site_at(a, z);
-      const P4 eq = bisector(sp, zp);
-      bis[v] = make_float4(eq.x, eq.y, eq.z, eq.w);
-      const P4 d = sub3(zp, sp);
-      bd2[v] = dot3(d, d);
-      // emptiness test terms: |n|_1, |w|, and the required depth beyond the
-      // plane (kCullMargin in distance, times |n|)
-      cull[v] = make_float4(fabsf(eq.x) + fabsf(eq.y) + fabsf(eq.z), fabsf(eq.w), kCullMargin * sqrtf(bd2[v]), 0.f);
-    }
-    __syncthreads();
 
-    // Full clipping of one batch: lane takes `pair` when `live`.
-    auto clip_batch = [&](long long pair, bool live) {
-      int status = kNoIntersection;
-      if (live) {
-        const int t = (int)(pair / a.TK);
-        const int4 tv = a.tets[t];
-        const P4 v[4] = {ld4(a.verts, tv.x), ld4(a.verts, tv.y), ld4(a.verts, tv.z), ld4(a.verts, tv.w)};
-        c.init_tet(sp, v);
-        auto plane_of = [&](int vv, int z, int& pid, P4& eq, float& d2z) {
-          (void)z;
-          pid = 4 + vv;
-          eq = ld4(bis, vv);
-          d2z = bd2[vv];
-        };
-        PairSums<ACC> S;
-        const bool acc = run_pair<MF, ACC>(a, c, nbr, plane_of, S);
-        status = c.status;
-        if (status == kTriangleOverflow && a.retry_list && a.tcap < a.tcap_full) {
-          // this launch's vertex cap is below the configured one: the pair is
-          // recomputed from scratch at the configured caps by the retry pass
-          const unsigned slot = atomicAdd(a.retry_count, 1u);
-          if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = (int)pair;
-          status = -1;
-        } else {
-          if (acc) accumulate<ACC>(a, s, S);
-          if (a.pair_status) a.pair_status[pair] = (uint8_t)status;
-          if (a.pair_vals) {
-            double* pv = a.pair_vals + 4 * pair;
-            pv[0] = acc ? (double)S.v : 0.0;
-            pv[1] = acc ? (double)S.x : 0.0;
-            pv[2] = acc ? (double)S.y : 0.0;
-            pv[3] = acc ? (double)S.z : 0.0;
+  // wave-uniform scheduling state
+  int site0 = -1, site1 = -1;         // site of each table slot
+  int feed_slot = 1;                  // slot of the site being fed into the queue
+  int cursor = 0, feed_end = 0;       // its pairs not yet through the emptiness test
+  unsigned qhead = 0, qtail = 0;      // queue ring positions
+  unsigned qsplit = 0;                // entries before qsplit belong to the other slot
+  bool no_more_sites = false;
+  unsigned n_culled = 0;
+
+  // lane state
+  int lst = kIdle;
+  int pv = 0, pslot = 0, ppair = 0;
+
+  for (;;) {
+    // (1) refill the queue up to 64 entries
+    while (qtail - qhead < 64) {
+      if (cursor < feed_end) {
+        __syncthreads();  // queue entries read by the previous trip
+        const int i = cursor + lane;
+        const bool live = i < feed_end;
+        const int pair = live ? a.pair_ids[i] : 0;
+        bool culled = false;
+        if (depth > 0) {
+          P4 v[4] = {};
+          float pmax = 0.f;
+          if (live) {
+            tet_corners(a, pair / a.TK, v);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) pmax = fmaxf(pmax, fmaxf(fabsf(v[j].x), fmaxf(fabsf(v[j].y), fabsf(v[j].z))));
           }
-        }
-      }
-      count_status(a, status, live);
-    };
-
-    // Pass over the site's pairs: pairs whose tet lies entirely beyond one of
-    // the first `depth` bisectors (every vertex deeper than the rounding
-    // bound of the plane evaluation plus kCullMargin) have an empty
-    // intersection with the cell and end here as no_intersection; the others
-    // are queued and clipped 64 at a time.
-    int queued = 0;
-    for (int base = pb; base < pe; base += 64) {
-      const int i = base + lane;
-      const bool live = i < pe;
-      const long long pair = live ? a.pair_ids[i] : 0;
-      bool culled = false;
-      if (depth > 0) {
-        P4 v[4] = {};
-        float pmax = 0.f;
-        if (live) {
-          const int4 tv = a.tets[(int)(pair / a.TK)];
-          v[0] = ld4(a.verts, tv.x);
-          v[1] = ld4(a.verts, tv.y);
-          v[2] = ld4(a.verts, tv.z);
-          v[3] = ld4(a.verts, tv.w);
+          const float4* eb = bis + feed_slot * K4;
+          const float2* cb = cterm + feed_slot * K4;
+          bool open = live;
+          for (int q = 0; q < depth; ++q) {
+            if (__ballot(open) == 0) break;
+            if (open) {
+              const float4 e = eb[q];
+              const float2 m = cb[q];
+              if (m.x < 0.f) {  // invalid neighbour: no test, as the clipping would stop there
+                open = false;
+              } else {
+                const float lim = -fmaf(m.x, pmax, m.y);
+                bool all = true;
 #pragma unroll
-          for (int j = 0; j < 4; ++j) pmax = fmaxf(pmax, fmaxf(fabsf(v[j].x), fmaxf(fabsf(v[j].y), fabsf(v[j].z))));
-        }
-        bool open = live;
-        for (int q = 0; q < depth; ++q) {
-          if (__ballot(open) == 0) break;
-          if (open) {
-            const float4 e = bis[q];
-            const float4 m = cull[q];
-            const float lim = -fmaf(kCullRound, fmaf(m.x, pmax, m.y), m.z);
-            bool all = true;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) all = all && (fmaf(e.z, v[j].z, fmaf(e.y, v[j].y, e.x * v[j].x)) + e.w < lim);
-            if (all) {
-              culled = true;
-              open = false;
+                for (int j = 0; j < 4; ++j)
+                  all = all && (fmaf(e.z, v[j].z, fmaf(e.y, v[j].y, e.x * v[j].x)) + e.w < lim);
+                if (all) {
+                  culled = true;
+                  open = false;
+                }
+              }
             }
           }
+          if (culled && a.pair_status) a.pair_status[pair] = (uint8_t)kNoIntersection;
+          if (culled && a.pair_vals) {
+            double* pvals = a.pair_vals + 4 * (size_t)pair;
+            pvals[0] = pvals[1] = pvals[2] = pvals[3] = 0.0;
+          }
+          n_culled += (unsigned)__popcll(__ballot(culled));
         }
-        if (culled && a.pair_status) a.pair_status[pair] = (uint8_t)kNoIntersection;
-        if (culled && a.pair_vals) {
-          double* pv = a.pair_vals + 4 * pair;
-          pv[0] = pv[1] = pv[2] = pv[3] = 0.0;
-        }
-        count_status(a, kNoIntersection, culled);
+        const bool keep = live && !culled;
+        const unsigned long long mask = __ballot(keep);
+        const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+        if (keep) queue[(qtail + (unsigned)below) % kQueue] = make_int2(pair, feed_slot);
+        qtail += (unsigned)__popcll(mask);
+        cursor += 64;
+        continue;
       }
-      const bool keep = live && !culled;
-      const unsigned long long mask = __ballot(keep);
-      const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
-      if (keep) surv[queued + below] = (int)pair;
-      queued += __popcll(mask);
+      if (no_more_sites) break;
+      // the other slot is free once no lane and no queued entry refers to it
+      const int other = feed_slot ^ 1;
+      if (__ballot(lst != kIdle && pslot == other) != 0 || qhead < qsplit) break;
+      int s = 0;
+      if (lane == 0) s = (int)atomicAdd(a.site_counter, 1u);
+      s = __builtin_amdgcn_readfirstlane(s);
+      if (s >= a.n_sites) {
+        no_more_sites = true;
+        break;
+      }
+      const int pb = a.pair_start[s], pe = a.pair_start[s + 1];
+      if (pb == pe) continue;
+      const P4 sp = site_at(a, s);
+      const int32_t* nbr = a.site_knn + (size_t)s * K;
+      __syncthreads();  // the slot's last readers are done
+      float4* eb = bis + other * K4;
+      float2* cb = cterm + other * K4;
+      float* db = bd2 + other * K4;
+      for (int v = lane; v < K; v += 64) {
+        const int z = nbr[v];
+        if ((unsigned)z >= (unsigned)a.n_sites) {  // clipping stops at such a neighbour
+          eb[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+          cb[v] = make_float2(-1.f, 0.f);
+          db[v] = kInvalidNbr;
+          continue;
+        }
+        const P4 zp = site_at(a, z);
+        const P4 eq = bisector(sp, zp);
+        const P4 d = sub3(zp, sp);
+        const float dd = dot3(d, d);
+        eb[v] = make_float4(eq.x, eq.y, eq.z, eq.w);
+        db[v] = dd;
+        // emptiness test terms: rounding bound of the plane evaluation
+        // (relative kCullRound of |n|_1 * max|x| + |w|) and the required depth
+        // beyond the plane (kCullMargin in distance, times |n|)
+        cb[v] = make_float2(kCullRound * (fabsf(eq.x) + fabsf(eq.y) + fabsf(eq.z)),
+                            fmaf(kCullRound, fabsf(eq.w), kCullMargin * sqrtf(dd)));
+      }
       __syncthreads();
-      if (queued >= 64) {
-        const int mine = surv[lane];
-        const int rest = lane + 64 < queued ? surv[lane + 64] : 0;
-        __syncthreads();
-        if (lane + 64 < queued) surv[lane] = rest;
-        queued -= 64;
-        __syncthreads();
-        clip_batch(mine, true);
+      if (other) site1 = s; else site0 = s;
+      feed_slot = other;
+      cursor = pb;
+      feed_end = pe;
+      qsplit = qtail;
+    }
+    __syncthreads();
+
+    // (2) idle lanes take queued pairs in queue order
+    const unsigned long long idle = __ballot(lst == kIdle);
+    const unsigned avail = qtail - qhead;
+    const unsigned rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0));
+    if (lst == kIdle && rank < avail) {
+      const int2 q = queue[(qhead + rank) % kQueue];
+      ppair = q.x;
+      pslot = q.y;
+      pv = 0;
+      lst = kClipping;
+      P4 v[4];
+      tet_corners(a, ppair / a.TK, v);
+      c.ps.bis = bis + pslot * K4;
+      c.init_tet(site_at(a, pslot ? site1 : site0), v);
+    }
+    qhead += min((unsigned)__popcll(idle), avail);
+    const int n_clip = __popcll(__ballot(lst == kClipping));
+    const int n_wait = __popcll(__ballot(lst == kWaiting));
+    if (n_clip + n_wait == 0) {
+      if (no_more_sites && cursor >= feed_end && qhead == qtail) break;
+      continue;
+    }
+
+    // Result of a finished pair: retry entry, or accumulation and records.
+    auto finish = [&](const PairSums<ACC>& S, bool acc) {
+      const int status = c.status;
+      if (status == kTriangleOverflow && a.retry_list && a.tcap < a.tcap_full) {
+        // this launch's vertex cap is below the configured one: the pair is
+        // recomputed from scratch at the configured caps by the retry pass
+        const unsigned slot = atomicAdd(a.retry_count, 1u);
+        if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = ppair;
+      } else {
+        if (acc) accumulate<ACC>(a, pslot ? site1 : site0, S);
+        if (a.pair_status) a.pair_status[ppair] = (uint8_t)status;
+        if (a.pair_vals) {
+          double* pvals = a.pair_vals + 4 * (size_t)ppair;
+          pvals[0] = acc ? (double)S.v : 0.0;
+          pvals[1] = acc ? (double)S.x : 0.0;
+          pvals[2] = acc ? (double)S.y : 0.0;
+          pvals[3] = acc ? (double)S.z : 0.0;
+        }
+        atomicAdd(&hist[status], 1u);
+      }
+      lst = kIdle;
+    };
+
+    if (n_wait > 0 && (n_wait >= kMomentsGroup || n_wait >= n_clip)) {
+      // (3a) volume and moments of the waiting lanes' polytopes, together
+      if (lst == kWaiting) {
+        PairSums<ACC> S;
+        if (MF == 0)
+          c.template moments_ref<ACC>(S);
+        else
+          c.template moments_robust<ACC>(S);
+        bool acc = false;
+        // vertices formed by parallel planes (degenerate input) give
+        // non-finite sums; such a pair is not accumulated
+        if (!(isfinite((double)S.v) && isfinite((double)S.x) && isfinite((double)S.y) && isfinite((double)S.z)))
+          c.status = kNeedsExactPredicates;
+        else if (fabs((double)S.v) < 0.1)
+          c.status = kNoIntersection;
+        else
+          acc = true;
+        finish(S, acc);
+      }
+    } else if (lst == kClipping) {
+      // (3b) one bisector for every clipping lane: kernel A's loop
+      // (voronoi.cu:679-700) one trip at a time
+      bool done = false, fin = false;
+      if (K == 0) {
+        c.status = kSecurityRadiusNotReached;
+        done = fin = true;
+      } else {
+        const float d2z = bd2[pslot * K4 + pv];
+        if (d2z == kInvalidNbr) {
+          c.status = kInconsistentBoundary;
+          done = true;
+        } else {
+          c.clip(4 + pv, ld4(c.ps.bis, pv), d2z);
+          if (c.security_reached(d2z)) {
+            done = fin = true;
+          } else if (c.status != kSuccess) {
+            done = true;
+          } else if (++pv == K) {
+            c.status = kSecurityRadiusNotReached;
+            done = fin = true;
+          }
+        }
+      }
+      if (done) {
+        if (fin && c.status != kNoIntersection) {
+          lst = kWaiting;
+        } else {
+          PairSums<ACC> S;
+          finish(S, false);
+        }
       }
     }
-    if (queued > 0) {
-      const int mine = lane < queued ? surv[lane] : 0;
-      clip_batch(mine, lane < queued);
-    }
+  }
+  __syncthreads();
+  if (a.status_hist && lane < 7) {
+    const unsigned n = hist[lane] + (lane == kNoIntersection ? n_culled : 0u);
+    if (n) atomicAdd(a.status_hist + lane, (unsigned long long)n);
   }
 }
 
@@ -816,8 +956,8 @@ __global__ void k_moments_out(int n, int acc_mode, const float* acc_f32, const u
 
 size_t clip_plain_lds(int pcap, int tcap) { return (size_t)64 * (16 * pcap + 4 * tcap + pcap); }
 size_t clip_site_lds(int K, int tcap) {
-  return (size_t)16 * K + 4 * (size_t)((K + 3) & ~3) + 16 * 4 * 64 + (size_t)8 * tcap * 64 + (size_t)(4 + K) * 64 +
-         (size_t)16 * K + 4 * 128;
+  size_t off[8];
+  return site_lds_layout(K, tcap, off);
 }
 
 template <class Kern>
@@ -825,6 +965,24 @@ static hipError_t launch_lds(Kern k, dim3 grid, dim3 block, size_t lds, hipStrea
   hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k, grid, block, lds, st, a);
+  return hipGetLastError();
+}
+
+// The site-major kernel is persistent (waves take sites from a counter): by
+// default one workgroup per slot the device can keep resident.
+template <class Kern>
+static hipError_t launch_site(Kern k, int n_blocks, size_t lds, hipStream_t st, const ClipArgs& a) {
+  hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  if (n_blocks <= 0) {
+    int per_cu = 0, dev = 0, cus = 0;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, lds)) != hipSuccess) return e;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    n_blocks = std::max(1, per_cu * cus);
+  }
+  n_blocks = std::max(1, std::min(n_blocks, a.n_sites));
+  hipLaunchKernelGGL(k, dim3((unsigned)n_blocks), dim3(64), lds, st, a);
   return hipGetLastError();
 }
 
@@ -838,12 +996,12 @@ hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, i
     if (moments == 1 && acc_mode == 0) return launch_lds(k_clip_plain<1, 0>, grid, block, lds, st, a);
     return launch_lds(k_clip_plain<1, 1>, grid, block, lds, st, a);
   }
+  if (!a.site_counter) return hipErrorInvalidValue;
   const size_t lds = clip_site_lds(a.K, a.tcap);
-  const dim3 grid((unsigned)n_blocks), block(64);
-  if (moments == 0 && acc_mode == 0) return launch_lds(k_clip_site<0, 0>, grid, block, lds, st, a);
-  if (moments == 0 && acc_mode == 1) return launch_lds(k_clip_site<0, 1>, grid, block, lds, st, a);
-  if (moments == 1 && acc_mode == 0) return launch_lds(k_clip_site<1, 0>, grid, block, lds, st, a);
-  return launch_lds(k_clip_site<1, 1>, grid, block, lds, st, a);
+  if (moments == 0 && acc_mode == 0) return launch_site(k_clip_site<0, 0>, n_blocks, lds, st, a);
+  if (moments == 0 && acc_mode == 1) return launch_site(k_clip_site<0, 1>, n_blocks, lds, st, a);
+  if (moments == 1 && acc_mode == 0) return launch_site(k_clip_site<1, 0>, n_blocks, lds, st, a);
+  return launch_site(k_clip_site<1, 1>, n_blocks, lds, st, a);
 }
 
 hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count, int* start, int* cursor,

@@ -12,6 +12,7 @@ struct ClipArgs {
   int K;
   const float4* verts;       // (x, y, z, _)
   const int4* tets;          // 4 vertex ids, positively oriented
+  const float4* tet_xyz;     // site-major form: per tet its corners, 3 float4 (12 floats)
   const int32_t* tet_knn;    // n_tets x TK candidate sites
   int TK;
   long long n_pairs;         // n_tets * TK
@@ -24,6 +25,7 @@ struct ClipArgs {
   int* retry_list;           // optional, n_pairs entries
   unsigned* retry_count;     // with retry_list
   int cull_depth;            // site-major form: bisectors tried by the emptiness test (0 = off)
+  unsigned* site_counter;    // site-major form: next site to take, zero before the launch
   const int* pair_list;      // tet-major form: optional list of pairs to process
   const unsigned* pair_list_count;
   float* acc_f32;            // n_sites x 4 (vol, mx, my, mz), FP32 accumulation
@@ -38,8 +40,8 @@ size_t clip_site_lds(int K, int tcap);
 
 // form: 0 tet-major, 1 site-major; moments: 0 reference, 1 robust;
 // acc_mode: 0 FP32 atomics, 1 FP64 pair sums + fixed-point site sums.
-// n_blocks: workgroups of the site-major form, and of the tet-major form when
-// it runs over a.pair_list.
+// n_blocks: workgroups of the site-major form (0 = as many as stay resident),
+// and of the tet-major form when it runs over a.pair_list.
 hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, int n_blocks, hipStream_t st);
 
 // Groups the pairs of an n_tets x TK candidate table by site: site s owns

@@ -69,6 +69,7 @@ struct vd_ctx {
   float mesh_lo[3] = {0, 0, 0}, mesh_hi[3] = {0, 0, 0};
 
   DevBuf<float4> verts, sites;
+  DevBuf<float4> tet_xyz;  // per tet its 12 corner coordinates (3 float4)
   DevBuf<int4> tets;
   DevBuf<int4> offsets;
   int n_offsets = 0;
@@ -90,6 +91,7 @@ struct vd_ctx {
   DevBuf<unsigned long long> hist;
   DevBuf<int> retry_list;
   DevBuf<unsigned> retry_count;
+  DevBuf<unsigned> site_counter;
 
   // HIP events of the phases of the last kRing iterations: slot = iteration % kRing;
   // [0] start, [1] after kNN, [2] after pair grouping, [3] after clip, [4] after update
@@ -136,7 +138,7 @@ int check_config(vd_ctx* ctx, const vd_config& c) {
   // the tet-major kernel also serves the site-major form's overflow retries
   if (vd::clip_plain_lds(c.pcap, c.tcap) > 160 * 1024)
     return ctx->fail(VD_ERR_ARG, "pcap/tcap too large for the tet-major kernel's LDS");
-  const int fast_tcap = c.site_tcap > 0 ? std::min(c.site_tcap, c.tcap) : std::min(c.tcap, 32);
+  const int fast_tcap = c.site_tcap > 0 ? std::min(c.site_tcap, c.tcap) : std::min(c.tcap, 16);
   if (c.form == VD_FORM_SITE_MAJOR && vd::clip_site_lds(c.site_k, fast_tcap) > 160 * 1024)
     return ctx->fail(VD_ERR_ARG, "site_k/site_tcap too large for the site-major form's LDS");
   if (c.cull_depth < 0 || c.cull_depth > 256)
@@ -256,7 +258,7 @@ void vd_ctx_destroy(vd_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  ctx->verts.release(); ctx->sites.release(); ctx->tets.release(); ctx->offsets.release();
+  ctx->verts.release(); ctx->sites.release(); ctx->tet_xyz.release(); ctx->tets.release(); ctx->offsets.release();
   ctx->shape.release(); ctx->g_count.release(); ctx->g_start.release(); ctx->g_cursor.release();
   ctx->g_cell_of.release(); ctx->g_sorted_ids.release(); ctx->g_sorted_pts.release();
   ctx->g_scan_tmp.release(); ctx->p_scan_tmp.release();
@@ -265,7 +267,7 @@ void vd_ctx_destroy(vd_ctx* ctx) {
   ctx->p_site_start.release();
   ctx->acc_f32.release(); ctx->acc_fix.release(); ctx->moments.release();
   ctx->pair_status.release(); ctx->pair_vals.release(); ctx->hist.release();
-  ctx->retry_list.release(); ctx->retry_count.release();
+  ctx->retry_list.release(); ctx->retry_count.release(); ctx->site_counter.release();
   for (auto& row : ctx->ev)
     for (auto& e : row)
       if (e) (void)hipEventDestroy(e);
@@ -311,6 +313,15 @@ int vd_set_mesh(vd_ctx* ctx, const float* verts_xyz, int n_verts, const int32_t*
   VD_HIP(ctx->tets.ensure(std::max(n_tets, 1)), "alloc tets");
   if (n_tets > 0)
     VD_HIP(hipMemcpy(ctx->tets.p, tets, sizeof(int4) * (size_t)n_tets, hipMemcpyHostToDevice), "upload tets");
+  {
+    std::vector<float> rec((size_t)12 * std::max(n_tets, 1));
+    for (size_t t = 0; t < (size_t)n_tets; ++t)
+      for (int k = 0; k < 4; ++k)
+        for (int a = 0; a < 3; ++a) rec[12 * t + 3 * k + a] = verts_xyz[3 * (size_t)tets[4 * t + k] + a];
+    VD_HIP(ctx->tet_xyz.ensure((size_t)3 * std::max(n_tets, 1)), "alloc tet records");
+    VD_HIP(hipMemcpy(ctx->tet_xyz.p, rec.data(), sizeof(float) * rec.size(), hipMemcpyHostToDevice),
+           "upload tet records");
+  }
   ctx->n_verts = n_verts;
   ctx->n_tets = n_tets;
   ctx->have_knn = ctx->have_clip = false;
@@ -410,6 +421,7 @@ int vd_clip_step(vd_ctx* ctx) {
   a.K = ctx->K;
   a.verts = ctx->verts.p;
   a.tets = ctx->tets.p;
+  a.tet_xyz = ctx->tet_xyz.p;
   a.tet_knn = ctx->tet_knn.p;
   a.TK = ctx->TK;
   a.n_pairs = n_pairs;
@@ -436,8 +448,11 @@ int vd_clip_step(vd_ctx* ctx) {
     ctx->ev_pairs = true;
     a.pair_start = ctx->p_start.p;
     a.pair_ids = ctx->p_ids.p;
-    blocks = c.clip_blocks > 0 ? c.clip_blocks : n;
-    const int fast_tcap = c.site_tcap > 0 ? std::min(c.site_tcap, c.tcap) : std::min(c.tcap, 32);
+    blocks = c.clip_blocks;
+    VD_HIP(ctx->site_counter.ensure(1), "alloc site counter");
+    VD_HIP(vd::fill_u32(ctx->site_counter.p, 1, 0u, ctx->stream), "clear site counter");
+    a.site_counter = ctx->site_counter.p;
+    const int fast_tcap = c.site_tcap > 0 ? std::min(c.site_tcap, c.tcap) : std::min(c.tcap, 16);
     if (fast_tcap < c.tcap) {
       VD_HIP(ctx->retry_list.ensure(std::max(n_pairs, 1LL)), "alloc retry list");
       VD_HIP(ctx->retry_count.ensure(1), "alloc retry list");
