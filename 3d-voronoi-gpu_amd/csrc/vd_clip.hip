@@ -38,8 +38,8 @@ __device__ __forceinline__ int tri_at(uint32_t t, int k) { return (int)((t >> (8
 // Storage policies --------------------------------------------------------
 //
 // PlaneStoreLocal: planes [pcap][64] per lane, ids are positions (reference).
-// PlaneStoreShared: tet faces per lane (ids 0..3), bisector v at id 4+v in a
-//   table shared by the wave.  A separate counter nv keeps the reference's
+// PlaneStoreShared: tet faces of the lane (ids 0..3), bisector v at id 4+v in
+//   a table shared by the wave.  A separate counter nv keeps the reference's
 //   plane count so that vertex_overflow happens exactly where it would.
 
 struct PlaneStoreLocal {
@@ -49,19 +49,27 @@ struct PlaneStoreLocal {
   __device__ __forceinline__ void put(int i, const P4& p) { pl[i * 64 + lane] = make_float4(p.x, p.y, p.z, p.w); }
 };
 
-// Face planes of the lane (ids 0..3) and the wave's bisector table (id 4+v)
-// live in one LDS region; the id picks the address, so a plane fetch is one
-// ds_read_b128 without a branch.
+// Face planes of the lane (ids 0..3) in registers, the wave's bisector table
+// (id 4+v) in LDS: a fetch reads the table entry and selects.
 struct PlaneStoreShared {
-  float4* face;        // [4][64] per lane
+  P4 f0, f1, f2, f3;   // tet faces
   const float4* bis;   // [K] shared
   int lane;
   __device__ __forceinline__ P4 get(int i) const {
-    const float4* p = i < 4 ? face + (i * 64 + lane) : bis + (i - 4);
-    const float4 v = *p;
-    return {v.x, v.y, v.z, v.w};
+    const float4 b = bis[i < 4 ? 0 : i - 4];
+    float x = b.x, y = b.y, z = b.z, w = b.w;
+    x = i == 3 ? f3.x : x; y = i == 3 ? f3.y : y; z = i == 3 ? f3.z : z; w = i == 3 ? f3.w : w;
+    x = i == 2 ? f2.x : x; y = i == 2 ? f2.y : y; z = i == 2 ? f2.z : z; w = i == 2 ? f2.w : w;
+    x = i == 1 ? f1.x : x; y = i == 1 ? f1.y : y; z = i == 1 ? f1.z : z; w = i == 1 ? f1.w : w;
+    x = i == 0 ? f0.x : x; y = i == 0 ? f0.y : y; z = i == 0 ? f0.z : z; w = i == 0 ? f0.w : w;
+    return {x, y, z, w};
   }
-  __device__ __forceinline__ void put(int i, const P4& p) { face[i * 64 + lane] = make_float4(p.x, p.y, p.z, p.w); }
+  __device__ __forceinline__ void put(int i, const P4& p) {
+    if (i == 0) f0 = p;
+    if (i == 1) f1 = p;
+    if (i == 2) f2 = p;
+    if (i == 3) f3 = p;
+  }
 };
 
 // Triangle (= polytope vertex) storage policies.
@@ -108,6 +116,10 @@ struct Cell {
   int first;
   int status;
   P4 seed;
+  // cached form: slots whose vertex is not yet known to lie within the near
+  // ball of the current bisector (see clip), and the d2z of the last clip
+  uint32_t far;
+  float last_d2z;
 
   __device__ __forceinline__ uint32_t tri(int t) const { return ts.get(t); }
   __device__ __forceinline__ int nxt(int p) const { return nx[p * 64 + lane]; }
@@ -118,12 +130,17 @@ struct Cell {
     const P4 d = sub3(X, seed);
     return dot3(d, d);
   }
-  // Stores triangle t; the cached form also computes its vertex distance.
-  __device__ __forceinline__ void put_tri(int t, uint32_t v) {
-    if constexpr (TS::kCached)
-      ts.put(t, v, seed_d2(vertex(v)));
-    else
+  // Stores triangle t; the cached form also computes its vertex distance,
+  // which it returns.
+  __device__ __forceinline__ float put_tri(int t, uint32_t v) {
+    if constexpr (TS::kCached) {
+      const float d2 = seed_d2(vertex(v));
+      ts.put(t, v, d2);
+      return d2;
+    } else {
       ts.put(t, v, 0.f);
+      return 0.f;
+    }
   }
 
   // Tet start: planes of faces cbd, acd, bad, abc (voronoi.cu:163-201).
@@ -141,6 +158,8 @@ struct Cell {
     put_tri(3, pack3(0, 2, 3));
     nt = 4;
     first = kNoPlane;
+    far = 0xFu;
+    last_d2z = 0.f;
   }
 
   // Every vertex closer to the seed than half the distance to z
@@ -222,48 +241,103 @@ struct Cell {
 
   // Clip by plane eqn stored under id `pid` (voronoi.cu:443-486); the plane is
   // the bisector of the seed with a site at squared distance d2z.
-  __device__ void clip(int pid, const P4& eqn, float d2z) {
+  //
+  // Returns the outcome of the security-radius test that follows the clip in
+  // kernel A (d2z > 4 max_t |x_t - seed|^2) when the cached form knows it (1
+  // or 0), or -1 when the caller has to evaluate it.
+  __device__ int clip(int pid, const P4& eqn, float d2z) {
     if (nv >= pcap) {
       status = kVertexOverflow;
-      return;
+      return -1;
     }
     ++nv;
     nr = 0;
-    // With cached vertex distances: a vertex within 0.9 * |z - s| / 2 of the
-    // seed is closer to s than to z by at least 0.05 |z - s|, far beyond the
-    // FP32 error of the determinant, so the test's outcome (no conflict) is
-    // known without computing it.
     if constexpr (TS::kCached) {
-      // (1) candidates: vertices outside that ball (one bit per slot, tcap <= 32);
-      // (2) the determinant only for them, so a wave pays for the largest
-      //     candidate count rather than the largest vertex count;
-      // (3) the reference's swap loop replayed from the flags: a vertex's
-      //     flag depends on its planes only, so the resulting order is the
-      //     one the reference's in-loop tests produce.
+      // A vertex within 0.9 * |z - s| / 2 of the seed (d2 < 0.2025 d2z) is
+      // closer to s than to z by at least 0.05 |z - s|, far beyond the FP32
+      // error of the determinant, so the test's outcome (no conflict) is known
+      // without computing it.  d2z does not decrease along the neighbour list,
+      // so a vertex found inside that ball stays inside for the later
+      // bisectors: `far` keeps the slots not yet found inside.
+      // Security: a vertex blocks it when 4 d2 >= d2z (false for NaN, which
+      // the reference's fmaxf skips); only far vertices can block.
+      if (!(d2z >= last_d2z)) far = (1u << nt) - 1u;
+      last_d2z = d2z;
       const float near2 = 0.2025f * d2z;
-      uint32_t cand = 0;
-      for (int t = 0; t < nt; ++t)
-        if (!(ts.d2(t) < near2)) cand |= 1u << t;
-      uint32_t conf = 0;
-      while (cand) {
-        const int t = __builtin_ctz(cand);
-        cand &= cand - 1;
-        const uint32_t tt = tri(t);
-        const Minors m = vertex_minors(ps.get(tri_a(tt)), ps.get(tri_b(tt)), ps.get(tri_c(tt)));
-        if (conflict_det(m, eqn) > 0.f) conf |= 1u << t;
-      }
-      int i = 0;
-      while (i < nt) {
-        if ((conf >> i) & 1u) {
-          --nt;
-          ts.swap(i, nt);
-          const uint32_t bn = (conf >> nt) & 1u;
-          conf = (conf & ~((1u << i) | (1u << nt))) | (bn << i);
-          ++nr;
+      uint32_t cand = 0, block = 0;
+      for (uint32_t m = far; m; m &= m - 1) {
+        const int t = __builtin_ctz(m);
+        const float d = ts.d2(t);
+        if (d < near2) {
+          far &= ~(1u << t);
         } else {
-          ++i;
+          cand |= 1u << t;
+          if (4.f * d >= d2z) block |= 1u << t;
         }
       }
+      // the determinant only for candidates: a wave pays for the largest
+      // candidate count rather than the largest vertex count
+      uint32_t conf = 0;
+      for (uint32_t m = cand; m; m &= m - 1) {
+        const int t = __builtin_ctz(m);
+        const uint32_t tt = tri(t);
+        const Minors mn = vertex_minors(ps.get(tri_a(tt)), ps.get(tri_b(tt)), ps.get(tri_c(tt)));
+        if (conflict_det(mn, eqn) > 0.f) conf |= 1u << t;
+      }
+      const bool kept_secure = d2z > 0.f && (block & ~conf) == 0u;
+      if (conf == 0u) {
+        --nv;
+        return kept_secure ? 1 : 0;
+      }
+      // The reference's loop (ascending slot, a conflicting slot takes the
+      // last live triangle and is tested again) replayed from the flags: a
+      // flag depends on the triangle's planes only, so the resulting order is
+      // the one the in-loop tests produce; each trip removes one triangle.
+      while (conf) {
+        const int i = __builtin_ctz(conf);
+        --nt;
+        ++nr;
+        const uint32_t bi = 1u << i, bn = 1u << nt;
+        if (i != nt) {
+          ts.swap(i, nt);
+          conf = (conf & ~(bi | bn)) | ((conf & bn) ? bi : 0u);
+          far = (far & ~(bi | bn)) | ((far & bn) ? bi : 0u);
+        } else {
+          conf &= ~bi;
+          far &= ~bi;
+        }
+      }
+      if (nt == 0) {
+        status = kNoIntersection;
+        return -1;
+      }
+      cavity_boundary();
+      if (status != kSuccess) return -1;
+      if (first == kNoPlane) return kept_secure ? 1 : 0;
+      bool new_block = false;
+      int c = first;
+      int guard = 0;
+      do {
+        if (c == kNoPlane || guard++ > pcap) {
+          status = kInconsistentBoundary;
+          return -1;
+        }
+        if (nt + 1 >= tcap) {
+          status = kTriangleOverflow;
+          return -1;
+        }
+        const int n = nxt(c);
+        if (n == kNoPlane) {
+          status = kInconsistentBoundary;
+          return -1;
+        }
+        const float d = put_tri(nt, pack3(pid, c, n));
+        if (!(d < near2)) far |= 1u << nt;
+        new_block = new_block || (4.f * d >= d2z);
+        ++nt;
+        c = n;
+      } while (c != first);
+      return (kept_secure && !new_block) ? 1 : 0;
     } else {
       int i = 0;
       while (i < nt) {
@@ -277,38 +351,39 @@ struct Cell {
           ++i;
         }
       }
-    }
-    if (nr == 0) {
-      --nv;
-      return;
-    }
-    if (nt == 0) {
-      status = kNoIntersection;
-      return;
-    }
-    cavity_boundary();
-    if (status != kSuccess) return;
-    if (first == kNoPlane) return;
-    int c = first;
-    int guard = 0;
-    do {
-      if (c == kNoPlane || guard++ > pcap) {
-        status = kInconsistentBoundary;
-        return;
+      if (nr == 0) {
+        --nv;
+        return -1;
       }
-      if (nt + 1 >= tcap) {
-        status = kTriangleOverflow;
-        return;
+      if (nt == 0) {
+        status = kNoIntersection;
+        return -1;
       }
-      const int n = nxt(c);
-      if (n == kNoPlane) {
-        status = kInconsistentBoundary;
-        return;
-      }
-      put_tri(nt, pack3(pid, c, n));
-      ++nt;
-      c = n;
-    } while (c != first);
+      cavity_boundary();
+      if (status != kSuccess) return -1;
+      if (first == kNoPlane) return -1;
+      int c = first;
+      int guard = 0;
+      do {
+        if (c == kNoPlane || guard++ > pcap) {
+          status = kInconsistentBoundary;
+          return -1;
+        }
+        if (nt + 1 >= tcap) {
+          status = kTriangleOverflow;
+          return -1;
+        }
+        const int n = nxt(c);
+        if (n == kNoPlane) {
+          status = kInconsistentBoundary;
+          return -1;
+        }
+        put_tri(nt, pack3(pid, c, n));
+        ++nt;
+        c = n;
+      } while (c != first);
+      return -1;
+    }
   }
 
   // Volume and first moment: the reference's six-tet-per-vertex decomposition
@@ -426,8 +501,8 @@ __device__ __forceinline__ bool run_pair(const ClipArgs& a, Cell<PS, TS>& c, con
     P4 eq;
     float d2z;
     plane_of(v, z, pid, eq, d2z);
-    c.clip(pid, eq, d2z);
-    if (c.security_reached(d2z)) {
+    const int sec = c.clip(pid, eq, d2z);
+    if (sec >= 0 ? sec != 0 : c.security_reached(d2z)) {
       stopped = true;
       break;
     }
@@ -559,9 +634,9 @@ __host__ __device__ inline size_t site_lds_layout(int K, int tcap, size_t* off) 
   const size_t K4 = (size_t)((K + 3) & ~3);
   size_t o = 0;
   off[0] = o; o += 16 * 2 * K4;        // bis   float4 [2][K4]
-  off[1] = o; o += 8 * 2 * K4;         // cterm float2 [2][K4]
+  off[1] = o;                          // (unused)
   off[2] = o; o += 4 * 2 * K4;         // bd2   float  [2][K4]
-  off[3] = o; o += 16 * 4 * 64;        // face  float4 [4][64]
+  off[3] = o;                          // (unused)
   off[4] = o; o += 8 * (size_t)tcap * 64;  // tr uint2 [tcap][64]
   off[5] = o; o += 8 * kQueue;         // queue int2 [kQueue]
   off[6] = o; o += 4 * 8;              // hist  unsigned [8]
@@ -578,12 +653,10 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   size_t off[8];
   site_lds_layout(K, a.tcap, off);
   float4* bis = (float4*)(smem + off[0]);
-  float2* cterm = (float2*)(smem + off[1]);
   float* bd2 = (float*)(smem + off[2]);
   int2* queue = (int2*)(smem + off[5]);
   unsigned* hist = (unsigned*)(smem + off[6]);
   Cell<PlaneStoreShared, TriCached> c;
-  c.ps.face = (float4*)(smem + off[3]);
   c.ps.bis = bis;
   c.ps.lane = lane;
   c.ts.tr = (uint2*)(smem + off[4]);
@@ -626,17 +699,21 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
             for (int j = 0; j < 4; ++j) pmax = fmaxf(pmax, fmaxf(fabsf(v[j].x), fmaxf(fabsf(v[j].y), fabsf(v[j].z))));
           }
           const float4* eb = bis + feed_slot * K4;
-          const float2* cb = cterm + feed_slot * K4;
+          const float* db = bd2 + feed_slot * K4;
           bool open = live;
           for (int q = 0; q < depth; ++q) {
             if (__ballot(open) == 0) break;
             if (open) {
               const float4 e = eb[q];
-              const float2 m = cb[q];
-              if (m.x < 0.f) {  // invalid neighbour: no test, as the clipping would stop there
+              const float dd = db[q];
+              if (dd == kInvalidNbr) {  // invalid neighbour: no test, as the clipping would stop there
                 open = false;
               } else {
-                const float lim = -fmaf(m.x, pmax, m.y);
+                // rounding bound of the plane evaluation (relative kCullRound
+                // of |n|_1 * max|x| + |w|) plus the required depth beyond the
+                // plane (kCullMargin in distance, times |n|)
+                const float lim = -fmaf(kCullRound, fmaf(fabsf(e.x) + fabsf(e.y) + fabsf(e.z), pmax, fabsf(e.w)),
+                                        kCullMargin * sqrtf(dd));
                 bool all = true;
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
@@ -680,13 +757,11 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
       const int32_t* nbr = a.site_knn + (size_t)s * K;
       __syncthreads();  // the slot's last readers are done
       float4* eb = bis + other * K4;
-      float2* cb = cterm + other * K4;
       float* db = bd2 + other * K4;
       for (int v = lane; v < K; v += 64) {
         const int z = nbr[v];
         if ((unsigned)z >= (unsigned)a.n_sites) {  // clipping stops at such a neighbour
           eb[v] = make_float4(0.f, 0.f, 0.f, 0.f);
-          cb[v] = make_float2(-1.f, 0.f);
           db[v] = kInvalidNbr;
           continue;
         }
@@ -696,11 +771,6 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         const float dd = dot3(d, d);
         eb[v] = make_float4(eq.x, eq.y, eq.z, eq.w);
         db[v] = dd;
-        // emptiness test terms: rounding bound of the plane evaluation
-        // (relative kCullRound of |n|_1 * max|x| + |w|) and the required depth
-        // beyond the plane (kCullMargin in distance, times |n|)
-        cb[v] = make_float2(kCullRound * (fabsf(eq.x) + fabsf(eq.y) + fabsf(eq.z)),
-                            fmaf(kCullRound, fabsf(eq.w), kCullMargin * sqrtf(dd)));
       }
       __syncthreads();
       if (other) site1 = s; else site0 = s;
@@ -789,8 +859,8 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
           c.status = kInconsistentBoundary;
           done = true;
         } else {
-          c.clip(4 + pv, ld4(c.ps.bis, pv), d2z);
-          if (c.security_reached(d2z)) {
+          const int sec = c.clip(4 + pv, ld4(c.ps.bis, pv), d2z);
+          if (sec >= 0 ? sec != 0 : c.security_reached(d2z)) {
             done = fin = true;
           } else if (c.status != kSuccess) {
             done = true;
