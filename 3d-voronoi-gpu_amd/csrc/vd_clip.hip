@@ -620,14 +620,16 @@ constexpr float kInvalidNbr = -1.f;   // table mark of an out-of-range neighbour
 constexpr int kIdle = 0, kClipping = 1, kWaiting = 2;
 constexpr int kMomentsGroup = 24;
 
-// Corners of tet t from its record (x0 y0 z0 x1 | y1 z1 x2 y2 | z2 x3 y3 z3).
-__device__ __forceinline__ void tet_corners(const ClipArgs& a, int t, P4 v[4]) {
-  const float4* r = a.tet_xyz + 3 * (size_t)t;
-  const float4 q0 = r[0], q1 = r[1], q2 = r[2];
+// Corners of a tet from its record (x0 y0 z0 x1 | y1 z1 x2 y2 | z2 x3 y3 z3).
+__device__ __forceinline__ void corners_of(const float4& q0, const float4& q1, const float4& q2, P4 v[4]) {
   v[0] = {q0.x, q0.y, q0.z, 0.f};
   v[1] = {q0.w, q1.x, q1.y, 0.f};
   v[2] = {q1.z, q1.w, q2.x, 0.f};
   v[3] = {q2.y, q2.z, q2.w, 0.f};
+}
+__device__ __forceinline__ void tet_corners(const ClipArgs& a, int t, P4 v[4]) {
+  const float4* r = a.tet_xyz + 3 * (size_t)t;
+  corners_of(r[0], r[1], r[2], v);
 }
 
 __host__ __device__ inline size_t site_lds_layout(int K, int tcap, size_t* off) {
@@ -681,6 +683,13 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   int lst = kIdle;
   int pv = 0, pslot = 0, ppair = 0;
 
+  // Prefetch of the next chunk to test: its pair ids (stage 1), then its tet
+  // records (stage 2), each issued before a trip's clipping so that the loads
+  // complete under it.
+  int pf_stage = 0, pf_cursor = -1;  // wave-uniform
+  int pf_pair = 0;
+  float4 pf_r0 = {}, pf_r1 = {}, pf_r2 = {};
+
   for (;;) {
     // (1) refill the queue up to 64 entries
     while (qtail - qhead < 64) {
@@ -688,39 +697,60 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         __syncthreads();  // queue entries read by the previous trip
         const int i = cursor + lane;
         const bool live = i < feed_end;
-        const int pair = live ? a.pair_ids[i] : 0;
+        const bool pre = pf_stage > 0 && pf_cursor == cursor;
+        const int pair = pre ? pf_pair : (live ? a.pair_ids[i] : 0);
         bool culled = false;
         if (depth > 0) {
           P4 v[4] = {};
-          float pmax = 0.f;
+          float pmax = 0.f, vmin2 = 0.f;
           if (live) {
-            tet_corners(a, pair / a.TK, v);
+            if (pre && pf_stage == 2)
+              corners_of(pf_r0, pf_r1, pf_r2, v);
+            else
+              tet_corners(a, pair / a.TK, v);
+            const P4 fs = site_at(a, feed_slot ? site1 : site0);
+            vmin2 = INFINITY;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) pmax = fmaxf(pmax, fmaxf(fabsf(v[j].x), fmaxf(fabsf(v[j].y), fabsf(v[j].z))));
+            for (int j = 0; j < 4; ++j) {
+              pmax = fmaxf(pmax, fmaxf(fabsf(v[j].x), fmaxf(fabsf(v[j].y), fabsf(v[j].z))));
+              const P4 d = sub3(v[j], fs);
+              vmin2 = fminf(vmin2, dot3(d, d));
+            }
           }
           const float4* eb = bis + feed_slot * K4;
           const float* db = bd2 + feed_slot * K4;
           bool open = live;
-          for (int q = 0; q < depth; ++q) {
+          for (int q0 = 0; q0 < depth; q0 += 4) {
             if (__ballot(open) == 0) break;
             if (open) {
-              const float4 e = eb[q];
-              const float dd = db[q];
-              if (dd == kInvalidNbr) {  // invalid neighbour: no test, as the clipping would stop there
-                open = false;
-              } else {
-                // rounding bound of the plane evaluation (relative kCullRound
-                // of |n|_1 * max|x| + |w|) plus the required depth beyond the
-                // plane (kCullMargin in distance, times |n|)
-                const float lim = -fmaf(kCullRound, fmaf(fabsf(e.x) + fabsf(e.y) + fabsf(e.z), pmax, fabsf(e.w)),
-                                        kCullMargin * sqrtf(dd));
-                bool all = true;
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                  all = all && (fmaf(e.z, v[j].z, fmaf(e.y, v[j].y, e.x * v[j].x)) + e.w < lim);
-                if (all) {
-                  culled = true;
+              for (int j = 0; j < 4; ++j) {
+                const int q = q0 + j;
+                if (!open || q >= depth) break;
+                const float4 e = eb[q];
+                const float dd = db[q];
+                if (dd == kInvalidNbr) {  // invalid neighbour: no test, as the clipping would stop there
                   open = false;
+                } else if (4.f * vmin2 <= dd) {
+                  // the tet's nearest corner is within |z - s| / 2 of the seed,
+                  // so it is on the seed's side of this bisector and of every
+                  // later one (the neighbour distances do not decrease): no
+                  // later test can succeed
+                  open = false;
+                } else {
+                  // rounding bound of the plane evaluation (relative
+                  // kCullRound of |n|_1 * max|x| + |w|) plus the required depth
+                  // beyond the plane (kCullMargin in distance, times |n|)
+                  const float lim = -fmaf(kCullRound, fmaf(fabsf(e.x) + fabsf(e.y) + fabsf(e.z), pmax, fabsf(e.w)),
+                                          kCullMargin * __builtin_amdgcn_sqrtf(dd));
+                  bool all = true;
+#pragma unroll
+                  for (int k = 0; k < 4; ++k)
+                    all = all && (fmaf(e.z, v[k].z, fmaf(e.y, v[k].y, e.x * v[k].x)) + e.w < lim);
+                  if (all) {
+                    culled = true;
+                    open = false;
+                  }
                 }
               }
             }
@@ -738,6 +768,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         if (keep) queue[(qtail + (unsigned)below) % kQueue] = make_int2(pair, feed_slot);
         qtail += (unsigned)__popcll(mask);
         cursor += 64;
+        pf_stage = 0;
         continue;
       }
       if (no_more_sites) break;
@@ -802,6 +833,19 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
     if (n_clip + n_wait == 0) {
       if (no_more_sites && cursor >= feed_end && qhead == qtail) break;
       continue;
+    }
+    if (pf_stage == 1 && pf_cursor == cursor) {
+      if (depth > 0 && cursor + lane < feed_end) {
+        const float4* r = a.tet_xyz + 3 * (size_t)(pf_pair / a.TK);
+        pf_r0 = r[0];
+        pf_r1 = r[1];
+        pf_r2 = r[2];
+      }
+      pf_stage = 2;
+    } else if (pf_stage == 0 && cursor < feed_end) {
+      pf_cursor = cursor;
+      pf_pair = cursor + lane < feed_end ? a.pair_ids[cursor + lane] : 0;
+      pf_stage = 1;
     }
 
     // Result of a finished pair: retry entry, or accumulation and records.
