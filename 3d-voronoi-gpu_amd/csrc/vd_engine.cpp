@@ -94,9 +94,10 @@ struct vd_ctx {
   DevBuf<unsigned> site_counter;
 
   // HIP events of the phases of the last kRing iterations: slot = iteration % kRing;
-  // [0] start, [1] after kNN, [2] after pair grouping, [3] after clip, [4] after update
+  // [0] start, [1] after kNN, [2] after pair grouping, [3] after clip, [4] after update,
+  // [5] after the main clipping launch (before the overflow re-clipping)
   static constexpr int kRing = 256;
-  hipEvent_t ev[5][kRing] = {};
+  hipEvent_t ev[6][kRing] = {};
   int it_slot = 0;        // slot of the iteration being enqueued
   long long it_done = 0;  // iterations completed since the last timing reset
   bool ev_knn = false, ev_pairs = false, ev_clip = false, ev_update = false;
@@ -468,6 +469,7 @@ int vd_clip_step(vd_ctx* ctx) {
     ctx->ev_pairs = true;
   }
   VD_HIP(vd::clip_launch(a, c.form, c.moments, c.accum, blocks, ctx->stream), "clip");
+  VD_HIP(hipEventRecord(ctx->E(5), ctx->stream), "event");
   if (a.retry_list) {
     // pairs that overflowed the site-major launch's vertex cap, at the configured caps
     vd::ClipArgs r = a;
@@ -525,6 +527,7 @@ void slot_timing(vd_ctx* ctx, int slot, bool knn, bool pairs, bool clip, bool up
   if (knn) t->knn_ms += el(0, 1);
   if (knn && pairs) t->pairs_ms += el(1, 2);
   if (pairs && clip) t->clip_ms += el(2, 3);
+  if (pairs && clip) t->overflow_ms += el(5, 3);
   if (clip && update) t->update_ms += el(3, 4);
   if (knn && update) t->total_ms += el(0, 4);
 }

@@ -42,7 +42,7 @@ class Config(ctypes.Structure):
 
 
 class Timing(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_float) for n in ("knn_ms", "pairs_ms", "clip_ms", "update_ms", "total_ms")]
+    _fields_ = [(n, ctypes.c_float) for n in ("knn_ms", "pairs_ms", "clip_ms", "update_ms", "total_ms", "overflow_ms")]
 
 
 class KnInfo(ctypes.Structure):

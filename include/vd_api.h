@@ -80,6 +80,8 @@ typedef struct vd_timing {
   float clip_ms;   /* clipping + per-site accumulation */
   float update_ms; /* Lloyd update */
   float total_ms;  /* whole iteration */
+  float overflow_ms; /* part of clip_ms: re-clipping of the pairs that overflowed the
+                        site-major launch's vertex cap, at the configured caps */
 } vd_timing;
 
 typedef struct vd_ctx vd_ctx;
