@@ -656,7 +656,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   site_lds_layout(K, a.tcap, off);
   float4* bis = (float4*)(smem + off[0]);
   float* bd2 = (float*)(smem + off[2]);
-  int2* queue = (int2*)(smem + off[5]);
+  uint2* queue = (uint2*)(smem + off[5]);
   unsigned* hist = (unsigned*)(smem + off[6]);
   Cell<PlaneStoreShared, TriCached> c;
   c.ps.bis = bis;
@@ -673,7 +673,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   // wave-uniform scheduling state
   int site0 = -1, site1 = -1;         // site of each table slot
   int feed_slot = 1;                  // slot of the site being fed into the queue
-  int cursor = 0, feed_end = 0;       // its pairs not yet through the emptiness test
+  unsigned cursor = 0, feed_end = 0;  // its pairs not yet through the emptiness test
   unsigned qhead = 0, qtail = 0;      // queue ring positions
   unsigned qsplit = 0;                // entries before qsplit belong to the other slot
   bool no_more_sites = false;
@@ -681,13 +681,15 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
 
   // lane state
   int lst = kIdle;
-  int pv = 0, pslot = 0, ppair = 0;
+  int pv = 0, pslot = 0;
+  unsigned ppair = 0;
 
   // Prefetch of the next chunk to test: its pair ids (stage 1), then its tet
   // records (stage 2), each issued before a trip's clipping so that the loads
   // complete under it.
-  int pf_stage = 0, pf_cursor = -1;  // wave-uniform
-  int pf_pair = 0;
+  int pf_stage = 0;  // wave-uniform
+  unsigned pf_cursor = ~0u;
+  unsigned pf_pair = 0;
   float4 pf_r0 = {}, pf_r1 = {}, pf_r2 = {};
 
   for (;;) {
@@ -695,10 +697,10 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
     while (qtail - qhead < 64) {
       if (cursor < feed_end) {
         __syncthreads();  // queue entries read by the previous trip
-        const int i = cursor + lane;
+        const unsigned i = cursor + lane;
         const bool live = i < feed_end;
         const bool pre = pf_stage > 0 && pf_cursor == cursor;
-        const int pair = pre ? pf_pair : (live ? a.pair_ids[i] : 0);
+        const unsigned pair = pre ? pf_pair : (live ? a.pair_ids[i] : 0u);
         bool culled = false;
         if (depth > 0) {
           P4 v[4] = {};
@@ -707,7 +709,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
             if (pre && pf_stage == 2)
               corners_of(pf_r0, pf_r1, pf_r2, v);
             else
-              tet_corners(a, pair / a.TK, v);
+              tet_corners(a, (int)(pair / (unsigned)a.TK), v);
             const P4 fs = site_at(a, feed_slot ? site1 : site0);
             vmin2 = INFINITY;
 #pragma unroll
@@ -765,7 +767,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         const bool keep = live && !culled;
         const unsigned long long mask = __ballot(keep);
         const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
-        if (keep) queue[(qtail + (unsigned)below) % kQueue] = make_int2(pair, feed_slot);
+        if (keep) queue[(qtail + (unsigned)below) % kQueue] = make_uint2(pair, (unsigned)feed_slot);
         qtail += (unsigned)__popcll(mask);
         cursor += 64;
         pf_stage = 0;
@@ -782,7 +784,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         no_more_sites = true;
         break;
       }
-      const int pb = a.pair_start[s], pe = a.pair_start[s + 1];
+      const unsigned pb = a.pair_start[s], pe = a.pair_start[s + 1];
       if (pb == pe) continue;
       const P4 sp = site_at(a, s);
       const int32_t* nbr = a.site_knn + (size_t)s * K;
@@ -817,13 +819,13 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
     const unsigned avail = qtail - qhead;
     const unsigned rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0));
     if (lst == kIdle && rank < avail) {
-      const int2 q = queue[(qhead + rank) % kQueue];
+      const uint2 q = queue[(qhead + rank) % kQueue];
       ppair = q.x;
-      pslot = q.y;
+      pslot = (int)q.y;
       pv = 0;
       lst = kClipping;
       P4 v[4];
-      tet_corners(a, ppair / a.TK, v);
+      tet_corners(a, (int)(ppair / (unsigned)a.TK), v);
       c.ps.bis = bis + pslot * K4;
       c.init_tet(site_at(a, pslot ? site1 : site0), v);
     }
@@ -836,7 +838,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
     }
     if (pf_stage == 1 && pf_cursor == cursor) {
       if (depth > 0 && cursor + lane < feed_end) {
-        const float4* r = a.tet_xyz + 3 * (size_t)(pf_pair / a.TK);
+        const float4* r = a.tet_xyz + 3 * (size_t)(pf_pair / (unsigned)a.TK);
         pf_r0 = r[0];
         pf_r1 = r[1];
         pf_r2 = r[2];
@@ -996,10 +998,10 @@ __global__ __launch_bounds__(256) void k_pairs_count(const int32_t* tet_knn, lon
 }
 
 __global__ __launch_bounds__(256) void k_pairs_scatter(const int32_t* tet_knn, long long n_pairs, int n_sites,
-                                                        const int* start, int* cursor, int* pair_ids) {
+                                                        const unsigned* start, int* cursor, unsigned* pair_ids) {
   __shared__ int keys[kHashSlots];
   __shared__ int vals[kHashSlots];
-  __shared__ int offs[kHashSlots];
+  __shared__ unsigned offs[kHashSlots];
   for (int i = threadIdx.x; i < kHashSlots; i += blockDim.x) {
     keys[i] = kEmptySlot;
     vals[i] = 0;
@@ -1017,7 +1019,7 @@ __global__ __launch_bounds__(256) void k_pairs_scatter(const int32_t* tet_knn, l
   __syncthreads();
   for (int i = threadIdx.x; i < kHashSlots; i += blockDim.x) {
     if (keys[i] != kEmptySlot) {
-      offs[i] = start[keys[i]] + atomicAdd(cursor + keys[i], vals[i]);
+      offs[i] = start[keys[i]] + (unsigned)atomicAdd(cursor + keys[i], vals[i]);
       vals[i] = 0;
     }
   }
@@ -1028,8 +1030,8 @@ __global__ __launch_bounds__(256) void k_pairs_scatter(const int32_t* tet_knn, l
     const int s = tet_knn[p];
     if ((unsigned)s >= (unsigned)n_sites) continue;
     const int h = hash_find(keys, s);
-    const int pos = h >= 0 ? offs[h] + atomicAdd(&vals[h], 1) : start[s] + atomicAdd(cursor + s, 1);
-    pair_ids[pos] = (int)p;
+    const unsigned pos = h >= 0 ? offs[h] + (unsigned)atomicAdd(&vals[h], 1) : start[s] + (unsigned)atomicAdd(cursor + s, 1);
+    pair_ids[pos] = (unsigned)p;
   }
 }
 
@@ -1118,8 +1120,8 @@ hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, i
   return launch_site(k_clip_site<1, 1>, n_blocks, lds, st, a);
 }
 
-hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count, int* start, int* cursor,
-                         int* pair_ids, int* scan_tmp, hipStream_t st) {
+hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count, unsigned* start,
+                         int* cursor, unsigned* pair_ids, unsigned* scan_tmp, hipStream_t st) {
   const unsigned blocks = (unsigned)((n_pairs + kGroupPairs - 1) / kGroupPairs);
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(k_pairs_count, dim3(blocks), dim3(256), 0, st, tet_knn, n_pairs, n_sites, count);

@@ -17,16 +17,16 @@ struct ClipArgs {
   int TK;
   long long n_pairs;         // n_tets * TK
   int n_sites;
-  const int* pair_start;     // site-major form: n_sites + 1
-  const int* pair_ids;       // site-major form: n_pairs pair indices grouped by site
+  const unsigned* pair_start;  // site-major form: n_sites + 1
+  const unsigned* pair_ids;  // site-major form: n_pairs pair indices grouped by site
   int pcap, tcap;            // polytope caps (planes, vertices) of this launch
   int tcap_full;             // configured vertex cap (site-major form: pairs that
                              // overflow a smaller launch cap go to retry_list)
-  int* retry_list;           // optional, n_pairs entries
+  unsigned* retry_list;      // optional, n_pairs entries
   unsigned* retry_count;     // with retry_list
   int cull_depth;            // site-major form: bisectors tried by the emptiness test (0 = off)
   unsigned* site_counter;    // site-major form: next site to take, zero before the launch
-  const int* pair_list;      // tet-major form: optional list of pairs to process
+  const unsigned* pair_list; // tet-major form: optional list of pairs to process
   const unsigned* pair_list_count;
   float* acc_f32;            // n_sites x 4 (vol, mx, my, mz), FP32 accumulation
   unsigned long long* acc_fix;  // n_sites x 4, 2^-20 fixed point of FP64 pair sums
@@ -44,11 +44,11 @@ size_t clip_site_lds(int K, int tcap);
 // and of the tet-major form when it runs over a.pair_list.
 hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, int n_blocks, hipStream_t st);
 
-// Groups the pairs of an n_tets x TK candidate table by site: site s owns
-// pair_ids[start[s] .. start[s+1]).  count must be zero; count/cursor hold
+// Groups the pairs of an n_tets x TK candidate table (at most 2^32 - 1 pairs)
+// by site: site s owns pair_ids[start[s] .. start[s+1]).  count must be zero; count/cursor hold
 // n_sites ints, start n_sites + 1, scan_tmp scan_tmp_ints(n_sites).
-hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count, int* start, int* cursor,
-                         int* pair_ids, int* scan_tmp, hipStream_t st);
+hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count, unsigned* start,
+                         int* cursor, unsigned* pair_ids, unsigned* scan_tmp, hipStream_t st);
 
 hipError_t lloyd_update(float4* sites, int n, int acc_mode, const float* acc_f32, const unsigned long long* acc_fix,
                         hipStream_t st);

@@ -123,7 +123,7 @@ int vd_knn_dev(const float* ref_dev, int ref_nb, size_t ref_pitch, const float* 
                        q4.p);
   const vd::GridBufs gb = g.bufs();
   if ((e = vd::grid_build(gb, ref4.p, ref_nb, nullptr, nullptr, 3.1f, st))) return VD_ERR_HIP;
-  if ((e = vd::knn_query(gb, ref_nb, g.offsets.p, g.n_offsets, k, 2, query_nb, nullptr, nullptr, q4.p,
+  if ((e = vd::knn_query(gb, ref_nb, g.offsets.p, g.n_offsets, k, 2, query_nb, nullptr, nullptr, q4.p, nullptr,
                          knn_index_dev, (long long)index_pitch, st)))
     return VD_ERR_HIP;
   if ((e = hipStreamSynchronize(st))) return VD_ERR_HIP;
@@ -173,7 +173,7 @@ int vd_kn_solve(vd_kn* kn) {
   // neighbour lists of width kn->k; entries past n-1 stay -1
   if (!e)
     e = vd::knn_query(gb, kn->n, kn->grid.offsets.p, kn->grid.n_offsets, kn->k, 0, kn->n, nullptr, nullptr, nullptr,
-                      kn->by_orig.p, 0, kn->stream);
+                      nullptr, kn->by_orig.p, 0, kn->stream);
   if (!e) {
     const long long tot = (long long)kn->n * kn->k;
     hipLaunchKernelGGL(k_to_sorted, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, kn->stream, gb.sorted_ids,

@@ -65,6 +65,15 @@ struct vd_ctx {
 
   int n_verts = 0, n_tets = 0, n_sites = 0;
   int K = 0, TK = 0;
+  // batch of independent instances (vd_set_batch); empty: one instance made of
+  // the whole mesh and site set
+  struct Inst {
+    int v0, nv, t0, nt, s0, ns;
+  };
+  std::vector<Inst> inst;
+  bool batch() const { return inst.size() > 1; }
+  DevBuf<int> b_site_begin, b_cell_base, b_site_inst, b_tet_inst;
+  DevBuf<float> b_box;
   bool have_knn = false, have_clip = false;
   float mesh_lo[3] = {0, 0, 0}, mesh_hi[3] = {0, 0, 0};
 
@@ -75,13 +84,15 @@ struct vd_ctx {
   int n_offsets = 0;
   // grid
   DevBuf<vd::GridShape> shape;
-  DevBuf<int> g_count, g_start, g_cursor, g_cell_of, g_sorted_ids, g_scan_tmp, p_scan_tmp;
+  DevBuf<int> g_count, g_start, g_cursor, g_cell_of, g_sorted_ids, g_scan_tmp;
+  DevBuf<unsigned> p_scan_tmp;
   DevBuf<float4> g_sorted_pts;
   int max_cells = 0;
   // kNN results
   DevBuf<int32_t> site_knn, tet_knn;
   // site-major pairs
-  DevBuf<int> p_count, p_start, p_cursor, p_site_start, p_ids;
+  DevBuf<int> p_count, p_cursor;
+  DevBuf<unsigned> p_start, p_ids;
   // accumulators
   DevBuf<float> acc_f32;
   DevBuf<unsigned long long> acc_fix;
@@ -89,7 +100,7 @@ struct vd_ctx {
   DevBuf<uint8_t> pair_status;
   DevBuf<double> pair_vals;
   DevBuf<unsigned long long> hist;
-  DevBuf<int> retry_list;
+  DevBuf<unsigned> retry_list;
   DevBuf<unsigned> retry_count;
   DevBuf<unsigned> site_counter;
 
@@ -152,8 +163,23 @@ int check_config(vd_ctx* ctx, const vd_config& c) {
 }
 
 int ensure_grid(vd_ctx* ctx, int n) {
-  const int mc = vd::grid_max_cells(n, ctx->cfg.grid_density);
-  VD_HIP(ctx->shape.ensure(1), "alloc grid shape");
+  int mc = vd::grid_max_cells(n, ctx->cfg.grid_density);
+  if (ctx->batch()) {
+    // cells of instance i: cell_base[i] .. cell_base[i+1]
+    std::vector<int> base(ctx->inst.size() + 1, 0);
+    long long tot = 0;
+    for (size_t i = 0; i < ctx->inst.size(); ++i) {
+      base[i] = (int)tot;
+      tot += vd::grid_max_cells(std::max(ctx->inst[i].ns, 1), ctx->cfg.grid_density);
+    }
+    if (tot > 0x7fffffffLL) return ctx->fail(VD_ERR_ARG, "batch grid too large");
+    base.back() = (int)tot;
+    mc = (int)tot;
+    VD_HIP(ctx->b_cell_base.ensure(base.size()), "alloc grid");
+    VD_HIP(hipMemcpy(ctx->b_cell_base.p, base.data(), sizeof(int) * base.size(), hipMemcpyHostToDevice),
+           "upload grid");
+  }
+  VD_HIP(ctx->shape.ensure(std::max<size_t>(1, ctx->inst.size())), "alloc grid shape");
   VD_HIP(ctx->g_count.ensure(mc), "alloc grid");
   VD_HIP(ctx->g_start.ensure((size_t)mc + 1), "alloc grid");
   VD_HIP(ctx->g_cursor.ensure(mc), "alloc grid");
@@ -168,6 +194,13 @@ int ensure_grid(vd_ctx* ctx, int n) {
 vd::GridBufs grid_bufs(vd_ctx* ctx) {
   vd::GridBufs gb;
   gb.shape = ctx->shape.p;
+  if (ctx->batch()) {
+    gb.n_inst = (int)ctx->inst.size();
+    gb.cell_base = ctx->b_cell_base.p;
+    gb.pt_begin = ctx->b_site_begin.p;
+    gb.pt_inst = ctx->b_site_inst.p;
+    gb.box = ctx->b_box.p;
+  }
   gb.max_cells = ctx->max_cells;
   gb.count = ctx->g_count.p;
   gb.start = ctx->g_start.p;
@@ -259,13 +292,14 @@ void vd_ctx_destroy(vd_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  ctx->verts.release(); ctx->sites.release(); ctx->tet_xyz.release(); ctx->tets.release(); ctx->offsets.release();
+  ctx->verts.release(); ctx->sites.release(); ctx->tet_xyz.release();
+  ctx->b_site_begin.release(); ctx->b_cell_base.release(); ctx->b_site_inst.release(); ctx->b_tet_inst.release();
+  ctx->b_box.release(); ctx->tets.release(); ctx->offsets.release();
   ctx->shape.release(); ctx->g_count.release(); ctx->g_start.release(); ctx->g_cursor.release();
   ctx->g_cell_of.release(); ctx->g_sorted_ids.release(); ctx->g_sorted_pts.release();
   ctx->g_scan_tmp.release(); ctx->p_scan_tmp.release();
   ctx->site_knn.release(); ctx->tet_knn.release();
   ctx->p_count.release(); ctx->p_start.release(); ctx->p_cursor.release(); ctx->p_ids.release();
-  ctx->p_site_start.release();
   ctx->acc_f32.release(); ctx->acc_fix.release(); ctx->moments.release();
   ctx->pair_status.release(); ctx->pair_vals.release(); ctx->hist.release();
   ctx->retry_list.release(); ctx->retry_count.release(); ctx->site_counter.release();
@@ -293,13 +327,10 @@ int vd_ctx_set_config(vd_ctx* ctx, const vd_config* cfg) {
   return VD_OK;
 }
 
-int vd_set_mesh(vd_ctx* ctx, const float* verts_xyz, int n_verts, const int32_t* tets, int n_tets) {
-  if (!ctx || n_verts < 0 || n_tets < 0 || (n_verts && !verts_xyz) || (n_tets && !tets))
-    return ctx ? ctx->fail(VD_ERR_ARG, "vd_set_mesh: bad arguments") : VD_ERR_ARG;
-  VD_HIP(hipSetDevice(ctx->device), "set device");
-  for (long long i = 0; i < 4LL * n_tets; ++i)
-    if (tets[i] < 0 || tets[i] >= n_verts)
-      return ctx->fail(VD_ERR_PRECONDITION, "tet %lld refers to vertex %d of %d", i / 4, tets[i], n_verts);
+namespace {
+// Uploads vertices, tets (global vertex ids, validated by the caller) and the
+// per-tet corner records; sets the mesh bounding box.
+int load_mesh(vd_ctx* ctx, const float* verts_xyz, int n_verts, const int32_t* tets, int n_tets) {
   for (int a = 0; a < 3; ++a) {
     ctx->mesh_lo[a] = n_verts ? verts_xyz[a] : 0.f;
     ctx->mesh_hi[a] = n_verts ? verts_xyz[a] : 0.f;
@@ -328,10 +359,99 @@ int vd_set_mesh(vd_ctx* ctx, const float* verts_xyz, int n_verts, const int32_t*
   ctx->have_knn = ctx->have_clip = false;
   return VD_OK;
 }
+}  // namespace
+
+int vd_set_mesh(vd_ctx* ctx, const float* verts_xyz, int n_verts, const int32_t* tets, int n_tets) {
+  if (!ctx || n_verts < 0 || n_tets < 0 || (n_verts && !verts_xyz) || (n_tets && !tets))
+    return ctx ? ctx->fail(VD_ERR_ARG, "vd_set_mesh: bad arguments") : VD_ERR_ARG;
+  VD_HIP(hipSetDevice(ctx->device), "set device");
+  for (long long i = 0; i < 4LL * n_tets; ++i)
+    if (tets[i] < 0 || tets[i] >= n_verts)
+      return ctx->fail(VD_ERR_PRECONDITION, "tet %lld refers to vertex %d of %d", i / 4, tets[i], n_verts);
+  ctx->inst.clear();
+  return load_mesh(ctx, verts_xyz, n_verts, tets, n_tets);
+}
+
+int vd_set_batch(vd_ctx* ctx, int n_inst, const int32_t* n_verts, const int32_t* n_tets, const int32_t* n_sites,
+                 const float* verts_xyz, const int32_t* tets, const float* sites_xyz) {
+  if (!ctx || n_inst < 1 || !n_verts || !n_tets || !n_sites)
+    return ctx ? ctx->fail(VD_ERR_ARG, "vd_set_batch: bad arguments") : VD_ERR_ARG;
+  VD_HIP(hipSetDevice(ctx->device), "set device");
+  std::vector<vd_ctx::Inst> in((size_t)n_inst);
+  long long V = 0, T = 0, S = 0;
+  for (int i = 0; i < n_inst; ++i) {
+    if (n_verts[i] < 0 || n_tets[i] < 0 || n_sites[i] < 1)
+      return ctx->fail(VD_ERR_ARG, "vd_set_batch: instance %d has bad sizes", i);
+    in[i] = {(int)V, n_verts[i], (int)T, n_tets[i], (int)S, n_sites[i]};
+    V += n_verts[i];
+    T += n_tets[i];
+    S += n_sites[i];
+    if (V > 0x7fffffffLL || T > 0x7fffffffLL || S > 0x7fffffffLL) return ctx->fail(VD_ERR_ARG, "vd_set_batch: too large");
+  }
+  if ((V && !verts_xyz) || (T && !tets) || !sites_xyz) return ctx->fail(VD_ERR_ARG, "vd_set_batch: bad arguments");
+  // tets to global vertex ids; per-instance boxes; instance of every site and tet
+  std::vector<int32_t> gt((size_t)4 * T);
+  std::vector<float> box((size_t)6 * n_inst);
+  std::vector<int> site_begin((size_t)n_inst + 1), site_inst((size_t)S), tet_inst((size_t)std::max(T, 1LL));
+  for (int i = 0; i < n_inst; ++i) {
+    const vd_ctx::Inst& q = in[i];
+    for (long long j = 4LL * q.t0; j < 4LL * (q.t0 + q.nt); ++j) {
+      if (tets[j] < 0 || tets[j] >= q.nv)
+        return ctx->fail(VD_ERR_PRECONDITION, "instance %d: tet %lld refers to vertex %d of %d", i, j / 4 - q.t0,
+                         tets[j], q.nv);
+      gt[j] = tets[j] + q.v0;
+    }
+    for (int a = 0; a < 3; ++a) {
+      float lo = q.nv ? verts_xyz[3 * (size_t)q.v0 + a] : 0.f, hi = lo;
+      for (int v = q.v0; v < q.v0 + q.nv; ++v) {
+        lo = std::min(lo, verts_xyz[3 * (size_t)v + a]);
+        hi = std::max(hi, verts_xyz[3 * (size_t)v + a]);
+      }
+      box[6 * i + a] = lo;
+      box[6 * i + 3 + a] = hi;
+    }
+    site_begin[i] = q.s0;
+    for (int k = q.s0; k < q.s0 + q.ns; ++k) site_inst[k] = i;
+    for (int k = q.t0; k < q.t0 + q.nt; ++k) tet_inst[k] = i;
+  }
+  site_begin[n_inst] = (int)S;
+  int rc = load_mesh(ctx, verts_xyz, (int)V, gt.data(), (int)T);
+  if (rc != VD_OK) return rc;
+  rc = upload_xyz(ctx, ctx->sites, sites_xyz, (int)S);
+  if (rc != VD_OK) return rc;
+  ctx->n_sites = (int)S;
+  ctx->inst = in;
+  if (n_inst > 1) {
+    VD_HIP(ctx->b_site_begin.ensure(site_begin.size()), "alloc batch");
+    VD_HIP(ctx->b_site_inst.ensure(site_inst.size()), "alloc batch");
+    VD_HIP(ctx->b_tet_inst.ensure(tet_inst.size()), "alloc batch");
+    VD_HIP(ctx->b_box.ensure(box.size()), "alloc batch");
+    VD_HIP(hipMemcpy(ctx->b_site_begin.p, site_begin.data(), sizeof(int) * site_begin.size(), hipMemcpyHostToDevice),
+           "upload batch");
+    VD_HIP(hipMemcpy(ctx->b_site_inst.p, site_inst.data(), sizeof(int) * site_inst.size(), hipMemcpyHostToDevice),
+           "upload batch");
+    VD_HIP(hipMemcpy(ctx->b_tet_inst.p, tet_inst.data(), sizeof(int) * tet_inst.size(), hipMemcpyHostToDevice),
+           "upload batch");
+    VD_HIP(hipMemcpy(ctx->b_box.p, box.data(), sizeof(float) * box.size(), hipMemcpyHostToDevice), "upload batch");
+  }
+  rc = ensure_grid(ctx, std::max(ctx->n_sites, 1));
+  if (rc != VD_OK) return rc;
+  ctx->have_knn = ctx->have_clip = false;
+  return VD_OK;
+}
+
+int vd_get_batch(vd_ctx* ctx, int* n_inst) {
+  if (!ctx || !n_inst) return VD_ERR_ARG;
+  *n_inst = std::max<int>(1, (int)ctx->inst.size());
+  return VD_OK;
+}
 
 int vd_set_sites(vd_ctx* ctx, const float* sites_xyz, int n_sites) {
   if (!ctx || n_sites < 0 || (n_sites && !sites_xyz)) return ctx ? ctx->fail(VD_ERR_ARG, "vd_set_sites: bad arguments") : VD_ERR_ARG;
   VD_HIP(hipSetDevice(ctx->device), "set device");
+  if (ctx->batch() && n_sites != ctx->n_sites)
+    return ctx->fail(VD_ERR_STATE, "vd_set_sites: a batch holds %d sites, got %d", ctx->n_sites, n_sites);
+  if (!ctx->batch()) ctx->inst.clear();
   int rc = upload_xyz(ctx, ctx->sites, sites_xyz, n_sites);
   if (rc != VD_OK) return rc;
   rc = ensure_grid(ctx, std::max(n_sites, 1));
@@ -358,8 +478,20 @@ int vd_get_sites(vd_ctx* ctx, float* out) {
 
 int vd_get_dims(vd_ctx* ctx, int* k, int* tk, int* n_sites, int* n_tets) {
   if (!ctx) return VD_ERR_ARG;
-  const int K = std::max(0, std::min(ctx->cfg.site_k, ctx->n_sites - 1));
-  const int TK = std::max(0, std::min(tet_k_heuristic(ctx->n_sites, ctx->n_tets, ctx->cfg.tet_k), ctx->n_sites));
+  int K = std::max(0, std::min(ctx->cfg.site_k, ctx->n_sites - 1));
+  int TK = std::max(0, std::min(tet_k_heuristic(ctx->n_sites, ctx->n_tets, ctx->cfg.tet_k), ctx->n_sites));
+  if (ctx->batch()) {
+    // one K and TK for all instances: K from the smallest site set, TK the
+    // instances' common value (-1 when they differ)
+    K = ctx->cfg.site_k;
+    TK = -2;
+    for (const auto& q : ctx->inst) {
+      K = std::min(K, q.ns - 1);
+      const int t = std::max(0, std::min(tet_k_heuristic(q.ns, q.nt, ctx->cfg.tet_k), q.ns));
+      TK = TK == -2 || TK == t ? t : -1;
+    }
+    K = std::max(K, 0);
+  }
   if (k) *k = K;
   if (tk) *tk = TK;
   if (n_sites) *n_sites = ctx->n_sites;
@@ -372,6 +504,8 @@ int vd_knn_step(vd_ctx* ctx) {
   if (ctx->n_sites <= 0) return ctx->fail(VD_ERR_STATE, "vd_knn_step: no sites");
   VD_HIP(hipSetDevice(ctx->device), "set device");
   vd_get_dims(ctx, &ctx->K, &ctx->TK, nullptr, nullptr);
+  if (ctx->TK < 0)
+    return ctx->fail(VD_ERR_ARG, "the instances of the batch have different candidate counts; set tet_k");
   const int n = ctx->n_sites;
   VD_HIP(ctx->site_knn.ensure((size_t)n * std::max(ctx->K, 1)), "alloc site knn");
   VD_HIP(ctx->tet_knn.ensure((size_t)std::max(ctx->n_tets, 1) * std::max(ctx->TK, 1)), "alloc tet knn");
@@ -381,12 +515,12 @@ int vd_knn_step(vd_ctx* ctx) {
                         ctx->n_verts ? ctx->mesh_hi : nullptr, ctx->cfg.grid_density, ctx->stream),
          "grid build");
   if (ctx->K > 0)
-    VD_HIP(vd::knn_query(gb, n, ctx->offsets.p, ctx->n_offsets, ctx->K, 0, n, nullptr, nullptr, nullptr,
+    VD_HIP(vd::knn_query(gb, n, ctx->offsets.p, ctx->n_offsets, ctx->K, 0, n, nullptr, nullptr, nullptr, nullptr,
                          ctx->site_knn.p, 0, ctx->stream),
            "site knn");
   if (ctx->n_tets > 0 && ctx->TK > 0)
     VD_HIP(vd::knn_query(gb, n, ctx->offsets.p, ctx->n_offsets, ctx->TK, 1, ctx->n_tets, ctx->verts.p, ctx->tets.p,
-                         nullptr, ctx->tet_knn.p, 0, ctx->stream),
+                         nullptr, ctx->batch() ? ctx->b_tet_inst.p : nullptr, ctx->tet_knn.p, 0, ctx->stream),
            "tet knn");
   VD_HIP(hipEventRecord(ctx->E(1), ctx->stream), "event");
   ctx->ev_knn = true;
@@ -403,7 +537,7 @@ int vd_clip_step(vd_ctx* ctx) {
   const vd_config& c = ctx->cfg;
   const int n = ctx->n_sites;
   const long long n_pairs = (long long)ctx->n_tets * ctx->TK;
-  if (n_pairs > 0x7fffffffLL) return ctx->fail(VD_ERR_ARG, "more than 2^31 pairs in one context");
+  if (n_pairs > 0xffffffffLL) return ctx->fail(VD_ERR_ARG, "more than 2^32 - 1 pairs in one context");
   if (c.accum == VD_ACCUM_F32) {
     VD_HIP(ctx->acc_f32.ensure(4 * (size_t)n), "alloc acc");
     VD_HIP(vd::fill_u32(ctx->acc_f32.p, 4LL * n, 0u, ctx->stream), "clear acc");

@@ -67,12 +67,23 @@ __device__ __forceinline__ float dist2(const float4& r, const float4& q) {
 
 // Bounding box of the points (and of `extra`, a box given by the host), then
 // the grid shape: about n/density cells of equal size, at most max_cells.
-__global__ __launch_bounds__(kScanThreads) void k_grid_shape(const float4* pts, int n, float3 extra_lo,
+// One block per instance.
+__global__ __launch_bounds__(kScanThreads) void k_grid_shape(const float4* pts, int n, const int* pt_begin,
+                                                              const int* cell_base, const float* box, float3 extra_lo,
                                                               float3 extra_hi, int use_extra, float density,
-                                                              int max_cells, GridShape* out) {
+                                                              int max_cells, GridShape* shapes) {
   __shared__ float red[6][kScanThreads / 64];
+  const int inst = blockIdx.x;
+  const int p0 = pt_begin ? pt_begin[inst] : 0, p1 = pt_begin ? pt_begin[inst + 1] : n;
+  if (cell_base) max_cells = cell_base[inst + 1] - cell_base[inst];
+  if (box) {
+    extra_lo = make_float3(box[6 * inst], box[6 * inst + 1], box[6 * inst + 2]);
+    extra_hi = make_float3(box[6 * inst + 3], box[6 * inst + 4], box[6 * inst + 5]);
+    use_extra = 1;
+  }
+  n = p1 - p0;
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = p0 + threadIdx.x; i < p1; i += blockDim.x) {
     const float4 p = pts[i];
     lo[0] = fminf(lo[0], p.x); lo[1] = fminf(lo[1], p.y); lo[2] = fminf(lo[2], p.z);
     hi[0] = fmaxf(hi[0], p.x); hi[1] = fmaxf(hi[1], p.y); hi[2] = fmaxf(hi[2], p.z);
@@ -138,19 +149,21 @@ __global__ __launch_bounds__(kScanThreads) void k_grid_shape(const float4* pts, 
     // absolute slack for the cell-box distance bounds (rounding of the cell
     // assignment and of lo + c*h)
     g.slack = (float)(1e-3 * cs + 1e-6 * (fabs(l[0]) + fabs(l[1]) + fabs(l[2]) + 1024.0 * cs));
-    *out = g;
+    shapes[inst] = g;
   }
 }
 
-__global__ void k_grid_count(const float4* pts, int n, const GridShape* gs, int* cell_of, int* count) {
+__global__ void k_grid_count(const float4* pts, int n, const GridShape* gs, const int* pt_inst, const int* cell_base,
+                             int* cell_of, int* count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const GridShape g = *gs;
+  const int inst = pt_inst ? pt_inst[i] : 0;
+  const GridShape g = gs[inst];
   const float4 p = pts[i];
   const int cx = cell_coord(p.x, g.lo[0], g.inv_h, g.dim[0]);
   const int cy = cell_coord(p.y, g.lo[1], g.inv_h, g.dim[1]);
   const int cz = cell_coord(p.z, g.lo[2], g.inv_h, g.dim[2]);
-  const int c = cx + g.dim[0] * (cy + g.dim[1] * cz);
+  const int c = (cell_base ? cell_base[inst] : 0) + cx + g.dim[0] * (cy + g.dim[1] * cz);
   cell_of[i] = c;
   atomicAdd(count + c, 1);
 }
@@ -163,10 +176,11 @@ __global__ void k_grid_scatter(int n, const int* cell_of, const int* start, int*
 }
 
 // One thread per cell: order the cell's ids, then gather the points.
-__global__ void k_grid_sort(const GridShape* gs, const float4* pts, const int* start, int* sorted_ids,
+// n_cells < 0: the cells of the single instance gs[0].
+__global__ void k_grid_sort(const GridShape* gs, int n_cells, const float4* pts, const int* start, int* sorted_ids,
                             float4* sorted_pts) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= gs->ncell) return;
+  if (c >= (n_cells >= 0 ? n_cells : gs->ncell)) return;
   const int b = start[c], e = start[c + 1];
   for (int i = b + 1; i < e; ++i) {
     const int v = sorted_ids[i];
@@ -183,7 +197,10 @@ __global__ void k_grid_sort(const GridShape* gs, const float4* pts, const int* s
 // ---------------------------------------------------------------- query ---
 
 struct QueryArgs {
-  const GridShape* grid;
+  const GridShape* grid;  // per instance
+  const int* cell_base;   // per instance, or null (one instance)
+  const int* pt_inst;     // instance of every point, or null
+  const int* q_inst;      // modes 1 and 2: instance of every query, or null
   const float4* sorted_pts;
   const int* sorted_ids;
   const int* start;
@@ -287,15 +304,22 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
   uint64_t* stage = stage_all[wv];
   const int q = blockIdx.x * kQueryWaves + wv;
   if (q >= a.n_query) return;  // whole wave leaves together
-  const GridShape g = *a.grid;
 
   float4 qp;
   int self = -1;
   int out_row = q;
+  int inst = 0;
   if (a.mode == 0) {
     qp = a.sorted_pts[q];
     self = a.sorted_ids[q];
     out_row = self;
+    if (a.pt_inst) inst = a.pt_inst[self];
+  } else if (a.q_inst) {
+    inst = a.q_inst[q];
+  }
+  const GridShape g = a.grid[inst];
+  const int cbase = a.cell_base ? a.cell_base[inst] : 0;
+  if (a.mode == 0) {
   } else if (a.mode == 1) {
     const int4 t = a.tets[q];
     const float4 v0 = a.verts[t.x], v1 = a.verts[t.y], v2 = a.verts[t.z], v3 = a.verts[t.w];
@@ -350,7 +374,7 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
       const int cx = qc[0] + o.x, cy = qc[1] + o.y, cz = qc[2] + o.z;
       if (cx >= 0 && cx < g.dim[0] && cy >= 0 && cy < g.dim[1] && cz >= 0 && cz < g.dim[2]) {
         if (thr == ~0ull || !(cell_bound(g, qp, cx, cy, cz) > kth)) {
-          const int c = cx + g.dim[0] * (cy + g.dim[1] * cz);
+          const int c = cbase + cx + g.dim[0] * (cy + g.dim[1] * cz);
           st = a.start[c];
           cnt = a.start[c + 1] - st;
         }
@@ -389,12 +413,14 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
     }
   }
   if (!done) {
-    // offsets table exhausted without the stop test passing: scan everything
-    for (int f0 = 0; f0 < a.n_ref; f0 += 64) {
+    // offsets table exhausted without the stop test passing: scan all points
+    // of the instance
+    const int r0 = a.start[cbase], r1 = a.start[cbase + g.ncell];
+    for (int f0 = r0; f0 < r1; f0 += 64) {
       const int f = f0 + lane;
       uint64_t key = ~0ull;
       bool ok = false;
-      if (f < a.n_ref) {
+      if (f < r1) {
         // skip cells already visited: those inside the offset table's reach
         const float4 p = a.sorted_pts[f];
         const int id = a.sorted_ids[f];
@@ -446,21 +472,23 @@ constexpr int kScanTile = 1024 * kScanItems;
 
 __device__ __forceinline__ int scan_n(const int* n_dev, int n_max) { return n_dev ? min(*n_dev, n_max) : n_max; }
 
-__global__ __launch_bounds__(1024) void k_scan_tiles(const int* in, const int* n_dev, int n_max, int* out,
-                                                      int* cursor, int* tile_sum) {
-  __shared__ int part[1024];
+// T: int, or unsigned for totals up to 2^32 - 1.
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_tiles(const int* in, const int* n_dev, int n_max, T* out,
+                                                      int* cursor, T* tile_sum) {
+  __shared__ T part[1024];
   const int n = scan_n(n_dev, n_max);
   const long long base = (long long)blockIdx.x * kScanTile;
   if (base >= n && blockIdx.x > 0) return;
-  int v[kScanItems];
-  int s = 0;
+  T v[kScanItems];
+  T s = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
     const long long i = base + (long long)j * 1024 + threadIdx.x;
-    v[j] = i < n ? in[i] : 0;
+    v[j] = i < n ? (T)in[i] : (T)0;
   }
   // thread t owns items base + t*kScanItems .. ; reorder through LDS for that
-  __shared__ int tile[kScanTile];
+  __shared__ T tile[kScanTile];
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) tile[j * 1024 + threadIdx.x] = v[j];
   __syncthreads();
@@ -472,12 +500,12 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(const int* in, const int* n
   part[threadIdx.x] = s;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
-    const int x = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    const T x = threadIdx.x >= off ? part[threadIdx.x - off] : (T)0;
     __syncthreads();
     part[threadIdx.x] += x;
     __syncthreads();
   }
-  int run = part[threadIdx.x] - s;
+  T run = part[threadIdx.x] - s;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
     tile[threadIdx.x * kScanItems + j] = run;
@@ -495,32 +523,34 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(const int* in, const int* n
   if (threadIdx.x == 1023) tile_sum[blockIdx.x] = part[1023];
 }
 
-__global__ __launch_bounds__(1024) void k_scan_top(const int* n_dev, int n_max, int* tile_sum) {
-  __shared__ int part[1024];
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_top(const int* n_dev, int n_max, T* tile_sum) {
+  __shared__ T part[1024];
   const int n = scan_n(n_dev, n_max);
   const int tiles = (n + kScanTile - 1) / kScanTile;
   const int per = (tiles + 1023) / 1024;
   const int b = threadIdx.x * per, e = min(tiles, b + per);
-  int s = 0;
+  T s = 0;
   for (int i = b; i < e; ++i) s += tile_sum[i];
   part[threadIdx.x] = s;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
-    const int x = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    const T x = threadIdx.x >= off ? part[threadIdx.x - off] : (T)0;
     __syncthreads();
     part[threadIdx.x] += x;
     __syncthreads();
   }
-  int run = part[threadIdx.x] - s;
+  T run = part[threadIdx.x] - s;
   for (int i = b; i < e; ++i) {
-    const int t = tile_sum[i];
+    const T t = tile_sum[i];
     tile_sum[i] = run;
     run += t;
   }
   if (threadIdx.x == 1023) tile_sum[tiles] = part[1023];
 }
 
-__global__ void k_scan_add(const int* n_dev, int n_max, int* out, const int* tile_sum) {
+template <class T>
+__global__ void k_scan_add(const int* n_dev, int n_max, T* out, const T* tile_sum) {
   const int n = scan_n(n_dev, n_max);
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] += tile_sum[i / kScanTile];
@@ -529,13 +559,22 @@ __global__ void k_scan_add(const int* n_dev, int n_max, int* out, const int* til
 
 int scan_tmp_ints(int n_max) { return (n_max + kScanTile - 1) / kScanTile + 2; }
 
-hipError_t scan_excl(const int* in, const int* n_dev, int n_max, int* out, int* cursor, int* tmp, hipStream_t st) {
+template <class T>
+static hipError_t scan_excl_t(const int* in, const int* n_dev, int n_max, T* out, int* cursor, T* tmp,
+                              hipStream_t st) {
   const int tiles = std::max(1, (n_max + kScanTile - 1) / kScanTile);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(1024), 0, st, in, n_dev, n_max, out, cursor, tmp);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, n_dev, n_max, tmp);
-  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)(((long long)n_max + 1 + 255) / 256)), dim3(256), 0, st, n_dev,
+  hipLaunchKernelGGL(k_scan_tiles<T>, dim3(tiles), dim3(1024), 0, st, in, n_dev, n_max, out, cursor, tmp);
+  hipLaunchKernelGGL(k_scan_top<T>, dim3(1), dim3(1024), 0, st, n_dev, n_max, tmp);
+  hipLaunchKernelGGL(k_scan_add<T>, dim3((unsigned)(((long long)n_max + 1 + 255) / 256)), dim3(256), 0, st, n_dev,
                      n_max, out, tmp);
   return hipGetLastError();
+}
+hipError_t scan_excl(const int* in, const int* n_dev, int n_max, int* out, int* cursor, int* tmp, hipStream_t st) {
+  return scan_excl_t<int>(in, n_dev, n_max, out, cursor, tmp, st);
+}
+hipError_t scan_excl(const int* in, const int* n_dev, int n_max, unsigned* out, int* cursor, unsigned* tmp,
+                     hipStream_t st) {
+  return scan_excl_t<unsigned>(in, n_dev, n_max, out, cursor, tmp, st);
 }
 
 std::vector<int4> knn_offset_table() {
@@ -565,28 +604,37 @@ hipError_t grid_build(const GridBufs& gb, const float4* pts, int n, const float*
                       float density, hipStream_t st) {
   const float3 elo = extra_lo ? make_float3(extra_lo[0], extra_lo[1], extra_lo[2]) : make_float3(0, 0, 0);
   const float3 ehi = extra_hi ? make_float3(extra_hi[0], extra_hi[1], extra_hi[2]) : make_float3(0, 0, 0);
-  hipLaunchKernelGGL(k_grid_shape, dim3(1), dim3(kScanThreads), 0, st, pts, n, elo, ehi, extra_lo ? 1 : 0, density,
-                     gb.max_cells, gb.shape);
+  if (gb.n_inst < 1 || (gb.n_inst > 1 && (!gb.cell_base || !gb.pt_begin || !gb.pt_inst))) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_grid_shape, dim3((unsigned)gb.n_inst), dim3(kScanThreads), 0, st, pts, n, gb.pt_begin,
+                     gb.cell_base, gb.box, elo, ehi, extra_lo ? 1 : 0, density, gb.max_cells, gb.shape);
   hipError_t e = fill_u32(gb.count, gb.max_cells, 0u, st);
   if (e != hipSuccess) return e;
   const int nb = (n + 255) / 256;
-  if (n > 0) hipLaunchKernelGGL(k_grid_count, dim3(nb), dim3(256), 0, st, pts, n, gb.shape, gb.cell_of, gb.count);
-  e = scan_excl(gb.count, &gb.shape->ncell, gb.max_cells, gb.start, gb.cursor, gb.scan_tmp, st);
+  if (n > 0)
+    hipLaunchKernelGGL(k_grid_count, dim3(nb), dim3(256), 0, st, pts, n, gb.shape, gb.pt_inst, gb.cell_base,
+                       gb.cell_of, gb.count);
+  // one instance: scan only the cells in use; a batch: all cells (unused
+  // ones are empty)
+  e = scan_excl(gb.count, gb.n_inst == 1 ? &gb.shape->ncell : nullptr, gb.max_cells, gb.start, gb.cursor,
+                gb.scan_tmp, st);
   if (e != hipSuccess) return e;
   if (n > 0)
     hipLaunchKernelGGL(k_grid_scatter, dim3(nb), dim3(256), 0, st, n, gb.cell_of, gb.start, gb.cursor,
                        gb.sorted_ids);
-  hipLaunchKernelGGL(k_grid_sort, dim3((gb.max_cells + 255) / 256), dim3(256), 0, st, gb.shape, pts, gb.start,
-                     gb.sorted_ids, gb.sorted_pts);
+  hipLaunchKernelGGL(k_grid_sort, dim3((gb.max_cells + 255) / 256), dim3(256), 0, st, gb.shape,
+                     gb.n_inst == 1 ? -1 : gb.max_cells, pts, gb.start, gb.sorted_ids, gb.sorted_pts);
   return hipGetLastError();
 }
 
 hipError_t knn_query(const GridBufs& gb, int n_ref, const int4* offsets, int n_offsets, int k, int mode,
-                     int n_query, const float4* verts, const int4* tets, const float4* qpts, int32_t* out,
-                     long long out_pitch, hipStream_t st) {
+                     int n_query, const float4* verts, const int4* tets, const float4* qpts, const int* q_inst,
+                     int32_t* out, long long out_pitch, hipStream_t st) {
   if (k < 1 || k > 256) return hipErrorInvalidValue;
   QueryArgs a;
   a.grid = gb.shape;
+  a.cell_base = gb.cell_base;
+  a.pt_inst = gb.pt_inst;
+  a.q_inst = q_inst;
   a.sorted_pts = gb.sorted_pts;
   a.sorted_ids = gb.sorted_ids;
   a.start = gb.start;

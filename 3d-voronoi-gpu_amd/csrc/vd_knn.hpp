@@ -19,17 +19,27 @@ struct GridShape {
   int ncell;
 };
 
-// Device buffers of one grid (all sized by the caller).
+// Device buffers of a grid over one point set, or over a batch of n_inst
+// independent point sets (instances) that share the buffers: instance i owns
+// the points pt_begin[i] .. pt_begin[i+1] and the cells cell_base[i] ..
+// cell_base[i+1], and its grid is shape[i].  A query only sees the points of
+// its own instance.  With n_inst == 1 the three tables may be null (one
+// instance owning all points and max_cells cells).
 struct GridBufs {
-  GridShape* shape;   // 1
-  int max_cells;
-  int* count;         // max_cells
-  int* start;         // max_cells + 1
-  int* cursor;        // max_cells
-  int* cell_of;       // n
-  int* sorted_ids;    // n
-  float4* sorted_pts; // n
-  int* scan_tmp;      // scan_tmp_ints(max_cells)
+  GridShape* shape = nullptr;      // n_inst
+  int n_inst = 1;
+  const int* cell_base = nullptr;  // n_inst + 1, or null
+  const int* pt_begin = nullptr;   // n_inst + 1, or null
+  const int* pt_inst = nullptr;    // instance of every point, or null
+  const float* box = nullptr;      // n_inst x (lo xyz, hi xyz): box the grid must cover besides its points, or null
+  int max_cells = 0;               // total cells (cell_base[n_inst])
+  int* count = nullptr;            // max_cells
+  int* start = nullptr;            // max_cells + 1
+  int* cursor = nullptr;           // max_cells
+  int* cell_of = nullptr;          // n
+  int* sorted_ids = nullptr;       // n
+  float4* sorted_pts = nullptr;    // n
+  int* scan_tmp = nullptr;         // scan_tmp_ints(max_cells)
 };
 
 // Exclusive scan of in[0..n) into out[0..n] (out[n] = total), n = *n_dev when
@@ -37,6 +47,9 @@ struct GridBufs {
 // tmp holds scan_tmp_ints(n_max) ints.
 int scan_tmp_ints(int n_max);
 hipError_t scan_excl(const int* in, const int* n_dev, int n_max, int* out, int* cursor, int* tmp, hipStream_t st);
+// The same with unsigned sums (totals up to 2^32 - 1).
+hipError_t scan_excl(const int* in, const int* n_dev, int n_max, unsigned* out, int* cursor, unsigned* tmp,
+                     hipStream_t st);
 
 std::vector<int4> knn_offset_table();
 int grid_max_cells(int n, float density);
@@ -45,7 +58,8 @@ int grid_max_cells(int n, float density);
 hipError_t fill_u32(void* p, long long n, uint32_t v, hipStream_t st);
 
 // Bins n points (x,y,z,_) into a grid over their bounding box, extended by the
-// box [extra_lo, extra_hi] when given.  Deterministic.
+// box [extra_lo, extra_hi] when given (single instance) or by gb.box (per
+// instance).  Deterministic.
 hipError_t grid_build(const GridBufs& gb, const float4* pts, int n, const float* extra_lo, const float* extra_hi,
                       float density, hipStream_t st);
 
@@ -55,9 +69,10 @@ hipError_t grid_build(const GridBufs& gb, const float4* pts, int n, const float*
 //   mode 1: queries are the centroids of tets[0..n_query).
 //   mode 2: queries are qpts[0..n_query).
 // out: row-major n_query x k (out_pitch == 0) or out[j*out_pitch + q].
-// Missing neighbours (k larger than the point count) are -1.
+// Missing neighbours (k larger than the point count) are -1.  q_inst: the
+// instance of every query of mode 1 or 2 (null: instance 0).
 hipError_t knn_query(const GridBufs& gb, int n_ref, const int4* offsets, int n_offsets, int k, int mode,
-                     int n_query, const float4* verts, const int4* tets, const float4* qpts, int32_t* out,
-                     long long out_pitch, hipStream_t st);
+                     int n_query, const float4* verts, const int4* tets, const float4* qpts, const int* q_inst,
+                     int32_t* out, long long out_pitch, hipStream_t st);
 
 }  // namespace vd

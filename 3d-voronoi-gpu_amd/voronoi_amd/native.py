@@ -71,6 +71,8 @@ def lib():
         L.vd_ctx_set_config.argtypes = [P, P]
         L.vd_set_mesh.argtypes = [P, P, I, P, I]
         L.vd_set_sites.argtypes = [P, P, I]
+        L.vd_set_batch.argtypes = [P, I, P, P, P, P, P, P]
+        L.vd_get_batch.argtypes = [P, P]
         L.vd_get_sites.argtypes = [P, P]
         for f in ("vd_knn_step", "vd_clip_step", "vd_update_step", "vd_sync"):
             getattr(L, f).argtypes = [P]
@@ -161,6 +163,18 @@ class Engine:
     def set_sites(self, sites):
         s = np.ascontiguousarray(sites, dtype=np.float32)
         self._ck(lib().vd_set_sites(self.h, _p(s), len(s)), "vd_set_sites")
+
+    def set_batch(self, problems):
+        """problems: sequence of (verts, tets, sites), tets with local vertex ids."""
+        nv = np.array([len(p[0]) for p in problems], dtype=np.int32)
+        nt = np.array([len(p[1]) for p in problems], dtype=np.int32)
+        ns = np.array([len(p[2]) for p in problems], dtype=np.int32)
+        v = np.ascontiguousarray(np.concatenate([np.asarray(p[0], np.float32).reshape(-1, 3) for p in problems]))
+        t = np.ascontiguousarray(np.concatenate([np.asarray(p[1], np.int32).reshape(-1, 4) for p in problems]))
+        st = np.ascontiguousarray(np.concatenate([np.asarray(p[2], np.float32).reshape(-1, 3) for p in problems]))
+        self._ck(lib().vd_set_batch(self.h, len(problems), _p(nv), _p(nt), _p(ns), _p(v), _p(t), _p(st)),
+                 "vd_set_batch")
+        self.batch_sites = np.concatenate([[0], np.cumsum(ns)])
 
     def dims(self):
         k, tk, ns, nt = (ctypes.c_int() for _ in range(4))

@@ -102,8 +102,23 @@ int vd_ctx_set_config(vd_ctx* ctx, const vd_config* cfg);
  * tet positively oriented (first vertex on the positive side of face c,b,d).
  * Replaces copy_tet_data (voronoi.cu:881-923). */
 int vd_set_mesh(vd_ctx* ctx, const float* verts_xyz, int n_verts, const int32_t* tets, int n_tets);
-/* Sites, xyz triples in input order (the engine never reorders them). */
+/* Sites, xyz triples in input order (the engine never reorders them).  After
+ * vd_set_batch, n_sites must be the batch's total (all instances, in order). */
 int vd_set_sites(vd_ctx* ctx, const float* sites_xyz, int n_sites);
+
+/* A batch of n_inst independent problems (instances) solved together: every
+ * step below runs once over all of them, and each instance's result is the one
+ * it would get alone.  Instance i has n_verts[i] vertices, n_tets[i] tets whose
+ * vertex ids are local to it (0-based), and n_sites[i] >= 1 sites; verts_xyz,
+ * tets and sites_xyz hold the instances one after another.  All instances use
+ * one K = min(site_k, min_i n_sites[i] - 1) and one TK, which must come out
+ * equal for every instance (set tet_k, or vd_knn_step fails).  Replaces a
+ * loop over compute_clipped_voro_diagram_GPU (voronoi.cu:925-1188), one mesh
+ * per call.  Results (vd_get_sites, vd_get_moments) are the instances'
+ * concatenated in order; vd_set_mesh ends the batch. */
+int vd_set_batch(vd_ctx* ctx, int n_inst, const int32_t* n_verts, const int32_t* n_tets, const int32_t* n_sites,
+                 const float* verts_xyz, const int32_t* tets, const float* sites_xyz);
+int vd_get_batch(vd_ctx* ctx, int* n_inst);
 int vd_get_sites(vd_ctx* ctx, float* sites_xyz);
 
 /* One Lloyd iteration in three steps, or n_iter whole iterations.

@@ -1,0 +1,79 @@
+"""Batches of independent instances in one context: every instance's kNN,
+moments and Lloyd result must be the ones it gets alone (bit for bit in the
+FP64 accumulation mode, whose site sums do not depend on the order of the
+atomics)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances():
+    from voronoi_amd import meshgen
+
+    out = []
+    for i, (n, ns) in enumerate([(6, 150), (9, 400), (7, 260), (11, 700), (5, 120)]):
+        out.append(meshgen.make_problem("cube", n, ns, seed=100 + i))
+    out.append(meshgen.make_problem("torus", 14, 300, seed=9))
+    return out
+
+
+@pytest.mark.parametrize("form", ["site", "tet"])
+def test_batch_equals_instances_alone(vdn, form):
+    probs = _instances()
+    cfg = dict(accum="f64", tet_k=32, form=form, record_pairs=0)
+    with vdn.Engine(**cfg) as e:
+        e.set_batch(probs)
+        k, tk, ns, nt = e.dims()
+        assert (k, tk) == (min(90, min(len(p[2]) for p in probs) - 1), 32)
+        assert ns == sum(len(p[2]) for p in probs) and nt == sum(len(p[1]) for p in probs)
+        e.knn_step()
+        bknn = e.site_knn()
+        btk = e.tet_knn()
+        e.clip_step()
+        bmom = e.moments()
+        e.update_step()
+        e.lloyd(1)
+        bsites = e.sites()
+        off = e.batch_sites
+    t0 = 0
+    for i, (v, t, s) in enumerate(probs):
+        a, b = off[i], off[i + 1]
+        with vdn.Engine(**cfg) as e:
+            e.set_mesh(v, t)
+            e.set_sites(s)
+            e.knn_step()
+            assert e.dims()[0] == k
+            np.testing.assert_array_equal(bknn[a:b] - a, e.site_knn())
+            np.testing.assert_array_equal(btk[t0:t0 + len(t)] - a, e.tet_knn())
+            e.clip_step()
+            np.testing.assert_array_equal(bmom[a:b], e.moments())
+            e.update_step()
+            e.lloyd(1)
+            np.testing.assert_array_equal(bsites[a:b], e.sites())
+        t0 += len(t)
+
+
+def test_batch_site_knn_matches_oracle(vdn, O):
+    probs = _instances()
+    with vdn.Engine(tet_k=32) as e:
+        e.set_batch(probs)
+        k = e.dims()[0]
+        e.knn_step()
+        bknn = e.site_knn()
+        off = e.batch_sites
+    for i, (_, _, s) in enumerate(probs):
+        got = bknn[off[i]:off[i + 1]] - off[i]
+        np.testing.assert_array_equal(got, O.site_knn(np.asarray(s, np.float32), k))
+
+
+def test_batch_needs_one_candidate_count(vdn):
+    from voronoi_amd import meshgen
+
+    # reference heuristic (voronoi.cu:936): 90 candidates for 600 sites on 384 tets, 20 for 100 on 6000
+    probs = [meshgen.make_problem("cube", 4, 600, seed=1), meshgen.make_problem("cube", 10, 100, seed=2)]
+    with vdn.Engine(tet_k=0) as e:
+        e.set_batch(probs)
+        assert e.dims()[1] == -1
+        with pytest.raises(RuntimeError, match="candidate counts"):
+            e.knn_step()
