@@ -112,7 +112,19 @@ struct vd_ctx {
   int it_slot = 0;        // slot of the iteration being enqueued
   long long it_done = 0;  // iterations completed since the last timing reset
   bool ev_knn = false, ev_pairs = false, ev_clip = false, ev_update = false;
+  bool graph_slot[kRing] = {};  // iteration replayed from the graph: whole-iteration time only
   hipEvent_t& E(int phase) { return ev[phase][it_slot]; }
+
+  // one captured Lloyd iteration (vd_lloyd_graph)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  bool capturing = false;
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+  }
 
   int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -135,6 +147,14 @@ struct vd_ctx {
   } while (0)
 
 namespace {
+
+// Phase event of the iteration being enqueued; none while a graph is being
+// captured (a replay is timed as a whole).
+hipError_t record_phase(vd_ctx* ctx, int phase) {
+  if (ctx->capturing) return hipSuccess;
+  return hipEventRecord(ctx->E(phase), ctx->stream);
+}
+
 
 int check_config(vd_ctx* ctx, const vd_config& c) {
   if (c.site_k < 1 || c.site_k > 256) return ctx->fail(VD_ERR_ARG, "site_k must be in [1,256], got %d", c.site_k);
@@ -290,6 +310,7 @@ int vd_ctx_create(vd_ctx** out, int device, const vd_config* cfg) {
 
 void vd_ctx_destroy(vd_ctx* ctx) {
   if (!ctx) return;
+  ctx->drop_graph();
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->verts.release(); ctx->sites.release(); ctx->tet_xyz.release();
@@ -314,6 +335,7 @@ const char* vd_ctx_error(const vd_ctx* ctx) { return ctx ? ctx->err.c_str() : "n
 
 int vd_ctx_set_config(vd_ctx* ctx, const vd_config* cfg) {
   if (!ctx || !cfg) return VD_ERR_ARG;
+  ctx->drop_graph();
   int rc = check_config(ctx, *cfg);
   if (rc != VD_OK) return rc;
   const bool grid_changed = cfg->grid_density != ctx->cfg.grid_density;
@@ -368,6 +390,7 @@ int vd_set_mesh(vd_ctx* ctx, const float* verts_xyz, int n_verts, const int32_t*
   for (long long i = 0; i < 4LL * n_tets; ++i)
     if (tets[i] < 0 || tets[i] >= n_verts)
       return ctx->fail(VD_ERR_PRECONDITION, "tet %lld refers to vertex %d of %d", i / 4, tets[i], n_verts);
+  ctx->drop_graph();
   ctx->inst.clear();
   return load_mesh(ctx, verts_xyz, n_verts, tets, n_tets);
 }
@@ -389,6 +412,7 @@ int vd_set_batch(vd_ctx* ctx, int n_inst, const int32_t* n_verts, const int32_t*
     if (V > 0x7fffffffLL || T > 0x7fffffffLL || S > 0x7fffffffLL) return ctx->fail(VD_ERR_ARG, "vd_set_batch: too large");
   }
   if ((V && !verts_xyz) || (T && !tets) || !sites_xyz) return ctx->fail(VD_ERR_ARG, "vd_set_batch: bad arguments");
+  ctx->drop_graph();
   // tets to global vertex ids; per-instance boxes; instance of every site and tet
   std::vector<int32_t> gt((size_t)4 * T);
   std::vector<float> box((size_t)6 * n_inst);
@@ -451,6 +475,7 @@ int vd_set_sites(vd_ctx* ctx, const float* sites_xyz, int n_sites) {
   VD_HIP(hipSetDevice(ctx->device), "set device");
   if (ctx->batch() && n_sites != ctx->n_sites)
     return ctx->fail(VD_ERR_STATE, "vd_set_sites: a batch holds %d sites, got %d", ctx->n_sites, n_sites);
+  if (n_sites != ctx->n_sites) ctx->drop_graph();
   if (!ctx->batch()) ctx->inst.clear();
   int rc = upload_xyz(ctx, ctx->sites, sites_xyz, n_sites);
   if (rc != VD_OK) return rc;
@@ -509,7 +534,7 @@ int vd_knn_step(vd_ctx* ctx) {
   const int n = ctx->n_sites;
   VD_HIP(ctx->site_knn.ensure((size_t)n * std::max(ctx->K, 1)), "alloc site knn");
   VD_HIP(ctx->tet_knn.ensure((size_t)std::max(ctx->n_tets, 1) * std::max(ctx->TK, 1)), "alloc tet knn");
-  VD_HIP(hipEventRecord(ctx->E(0), ctx->stream), "event");
+  VD_HIP(record_phase(ctx, (0)), "event");
   const vd::GridBufs gb = grid_bufs(ctx);
   VD_HIP(vd::grid_build(gb, ctx->sites.p, n, ctx->n_verts ? ctx->mesh_lo : nullptr,
                         ctx->n_verts ? ctx->mesh_hi : nullptr, ctx->cfg.grid_density, ctx->stream),
@@ -522,7 +547,7 @@ int vd_knn_step(vd_ctx* ctx) {
     VD_HIP(vd::knn_query(gb, n, ctx->offsets.p, ctx->n_offsets, ctx->TK, 1, ctx->n_tets, ctx->verts.p, ctx->tets.p,
                          nullptr, ctx->batch() ? ctx->b_tet_inst.p : nullptr, ctx->tet_knn.p, 0, ctx->stream),
            "tet knn");
-  VD_HIP(hipEventRecord(ctx->E(1), ctx->stream), "event");
+  VD_HIP(record_phase(ctx, (1)), "event");
   ctx->ev_knn = true;
   ctx->ev_pairs = ctx->ev_clip = ctx->ev_update = false;
   ctx->have_knn = true;
@@ -579,7 +604,7 @@ int vd_clip_step(vd_ctx* ctx) {
     VD_HIP(vd::pairs_by_site(ctx->tet_knn.p, n_pairs, n, ctx->p_count.p, ctx->p_start.p, ctx->p_cursor.p, ctx->p_ids.p,
                              ctx->p_scan_tmp.p, ctx->stream),
            "pairs by site");
-    VD_HIP(hipEventRecord(ctx->E(2), ctx->stream), "event");
+    VD_HIP(record_phase(ctx, (2)), "event");
     ctx->ev_pairs = true;
     a.pair_start = ctx->p_start.p;
     a.pair_ids = ctx->p_ids.p;
@@ -599,11 +624,11 @@ int vd_clip_step(vd_ctx* ctx) {
     a.tcap_full = c.tcap;
     a.cull_depth = c.cull_depth;
   } else {
-    VD_HIP(hipEventRecord(ctx->E(2), ctx->stream), "event");
+    VD_HIP(record_phase(ctx, (2)), "event");
     ctx->ev_pairs = true;
   }
   VD_HIP(vd::clip_launch(a, c.form, c.moments, c.accum, blocks, ctx->stream), "clip");
-  VD_HIP(hipEventRecord(ctx->E(5), ctx->stream), "event");
+  VD_HIP(record_phase(ctx, (5)), "event");
   if (a.retry_list) {
     // pairs that overflowed the site-major launch's vertex cap, at the configured caps
     vd::ClipArgs r = a;
@@ -614,7 +639,7 @@ int vd_clip_step(vd_ctx* ctx) {
     r.retry_count = nullptr;
     VD_HIP(vd::clip_launch(r, VD_FORM_TET_MAJOR, c.moments, c.accum, 256, ctx->stream), "clip retry");
   }
-  VD_HIP(hipEventRecord(ctx->E(3), ctx->stream), "event");
+  VD_HIP(record_phase(ctx, (3)), "event");
   ctx->ev_clip = true;
   ctx->have_clip = true;
   return VD_OK;
@@ -625,10 +650,13 @@ int vd_update_step(vd_ctx* ctx) {
   if (!ctx->have_clip) return ctx->fail(VD_ERR_STATE, "vd_update_step: run vd_clip_step first");
   VD_HIP(vd::lloyd_update(ctx->sites.p, ctx->n_sites, ctx->cfg.accum, ctx->acc_f32.p, ctx->acc_fix.p, ctx->stream),
          "update");
-  VD_HIP(hipEventRecord(ctx->E(4), ctx->stream), "event");
+  VD_HIP(record_phase(ctx, (4)), "event");
   ctx->ev_update = true;
-  ctx->it_slot = (ctx->it_slot + 1) % vd_ctx::kRing;
-  ctx->it_done++;
+  if (!ctx->capturing) {
+    ctx->graph_slot[ctx->it_slot] = false;
+    ctx->it_slot = (ctx->it_slot + 1) % vd_ctx::kRing;
+    ctx->it_done++;
+  }
   ctx->ev_knn = ctx->ev_pairs = ctx->ev_clip = false;
   ctx->have_knn = false;  // sites moved
   return VD_OK;
@@ -645,6 +673,119 @@ int vd_lloyd(vd_ctx* ctx, int n_iter) {
   return VD_OK;
 }
 
+int vd_lloyd_graph(vd_ctx* ctx, int n_iter) {
+  if (!ctx || n_iter < 0) return VD_ERR_ARG;
+  if (n_iter == 0) return VD_OK;
+  VD_HIP(hipSetDevice(ctx->device), "set device");
+  if (!ctx->graph_exec) {
+    // an eager iteration sizes every buffer, so that the captured one
+    // allocates nothing
+    int rc = vd_lloyd(ctx, 1);
+    if (rc != VD_OK) return rc;
+    if (--n_iter == 0) return VD_OK;
+    VD_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+    VD_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    ctx->capturing = true;
+    rc = vd_knn_step(ctx);
+    if (rc == VD_OK) rc = vd_clip_step(ctx);
+    if (rc == VD_OK) rc = vd_update_step(ctx);
+    ctx->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc != VD_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc != VD_OK ? rc : ctx->hip_fail(e, "end capture");
+    }
+    ctx->graph = g;
+    const hipError_t ei = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      ctx->drop_graph();
+      return ctx->hip_fail(ei, "instantiate graph");
+    }
+  }
+  for (int i = 0; i < n_iter; ++i) {
+    VD_HIP(hipEventRecord(ctx->E(0), ctx->stream), "event");
+    VD_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream), "graph launch");
+    VD_HIP(hipEventRecord(ctx->E(4), ctx->stream), "event");
+    ctx->graph_slot[ctx->it_slot] = true;
+    ctx->it_slot = (ctx->it_slot + 1) % vd_ctx::kRing;
+    ctx->it_done++;
+  }
+  ctx->have_knn = false;
+  ctx->have_clip = true;
+  ctx->ev_knn = ctx->ev_pairs = ctx->ev_clip = false;
+  return VD_OK;
+}
+
+int vd_autotune(vd_ctx* ctx, int iters, vd_config* chosen) {
+  if (!ctx || iters < 1) return VD_ERR_ARG;
+  if (ctx->n_sites <= 0) return ctx->fail(VD_ERR_STATE, "vd_autotune: no sites");
+  VD_HIP(hipSetDevice(ctx->device), "set device");
+  DevBuf<float4> saved;
+  VD_HIP(saved.ensure(ctx->n_sites), "alloc");
+  VD_HIP(hipMemcpyAsync(saved.p, ctx->sites.p, sizeof(float4) * ctx->n_sites, hipMemcpyDeviceToDevice, ctx->stream),
+         "save sites");
+  auto restore = [&]() -> int {
+    VD_HIP(hipMemcpyAsync(ctx->sites.p, saved.p, sizeof(float4) * ctx->n_sites, hipMemcpyDeviceToDevice, ctx->stream),
+           "restore sites");
+    VD_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+    ctx->have_knn = ctx->have_clip = false;
+    return VD_OK;
+  };
+  // device time of `iters` iterations (after one untimed) under cfg, or a
+  // negative value when cfg is rejected
+  auto trial = [&](const vd_config& cfg, double* ms) -> int {
+    *ms = -1.0;
+    if (check_config(ctx, cfg) != VD_OK) return VD_OK;
+    int rc = vd_ctx_set_config(ctx, &cfg);
+    if (rc == VD_OK) rc = restore();
+    if (rc == VD_OK) rc = vd_lloyd(ctx, 1);
+    if (rc == VD_OK) rc = vd_timing_reset(ctx);
+    if (rc == VD_OK) rc = vd_lloyd(ctx, iters);
+    vd_timing t{};
+    int n = 0;
+    if (rc == VD_OK) rc = vd_get_timing_sum(ctx, &t, &n);
+    if (rc != VD_OK) return rc;
+    *ms = n > 0 ? t.total_ms / n : -1.0;
+    return VD_OK;
+  };
+  vd_config best = ctx->cfg;
+  double best_ms = 0.0;
+  int rc = trial(best, &best_ms);
+  if (rc != VD_OK) return rc;
+  // coordinate search: one parameter at a time, the others at their best
+  const int tcaps[] = {12, 16, 20, 24};
+  const int depths[] = {8, 16, 32, 64};
+  const float densities[] = {2.0f, 3.1f, 5.0f};
+  for (int dim = 0; dim < 3; ++dim) {
+    const int nv = dim == 2 ? 3 : 4;
+    for (int j = 0; j < nv; ++j) {
+      vd_config c = best;
+      if (dim == 0) {
+        if (c.form != VD_FORM_SITE_MAJOR) break;
+        c.site_tcap = tcaps[j];
+      } else if (dim == 1) {
+        if (c.form != VD_FORM_SITE_MAJOR) break;
+        c.cull_depth = depths[j];
+      } else {
+        c.grid_density = densities[j];
+      }
+      if (std::memcmp(&c, &best, sizeof c) == 0) continue;
+      double ms = 0.0;
+      if ((rc = trial(c, &ms)) != VD_OK) return rc;
+      if (ms > 0.0 && ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+  }
+  if ((rc = vd_ctx_set_config(ctx, &best)) != VD_OK) return rc;
+  if ((rc = restore()) != VD_OK) return rc;
+  VD_HIP(vd_timing_reset(ctx) == VD_OK ? hipSuccess : hipErrorUnknown, "timing reset");
+  if (chosen) *chosen = best;
+  return VD_OK;
+}
+
 int vd_sync(vd_ctx* ctx) {
   if (!ctx) return VD_ERR_ARG;
   VD_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
@@ -658,6 +799,10 @@ void slot_timing(vd_ctx* ctx, int slot, bool knn, bool pairs, bool clip, bool up
     (void)hipEventElapsedTime(&ms, ctx->ev[a][slot], ctx->ev[b][slot]);
     return ms;
   };
+  if (ctx->graph_slot[slot]) {
+    if (update) t->total_ms += el(0, 4);
+    return;
+  }
   if (knn) t->knn_ms += el(0, 1);
   if (knn && pairs) t->pairs_ms += el(1, 2);
   if (pairs && clip) t->clip_ms += el(2, 3);

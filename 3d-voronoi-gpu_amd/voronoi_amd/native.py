@@ -77,6 +77,8 @@ def lib():
         for f in ("vd_knn_step", "vd_clip_step", "vd_update_step", "vd_sync"):
             getattr(L, f).argtypes = [P]
         L.vd_lloyd.argtypes = [P, I]
+        L.vd_lloyd_graph.argtypes = [P, I]
+        L.vd_autotune.argtypes = [P, I, P]
         L.vd_get_timing.argtypes = [P, P]
         L.vd_timing_reset.argtypes = [P]
         L.vd_get_timing_sum.argtypes = [P, P, P]
@@ -199,6 +201,16 @@ class Engine:
 
     def lloyd(self, n_iter=1):
         self._ck(lib().vd_lloyd(self.h, n_iter), "vd_lloyd")
+
+    def autotune(self, iters=2):
+        """Times candidate launch parameters and keeps the fastest; returns them."""
+        c = Config()
+        self._ck(lib().vd_autotune(self.h, iters, ctypes.byref(c)), "vd_autotune")
+        return {n: getattr(c, n) for n, _ in Config._fields_}
+
+    def lloyd_graph(self, n_iter=1):
+        """n_iter iterations replayed from one captured HIP graph."""
+        self._ck(lib().vd_lloyd_graph(self.h, n_iter), "vd_lloyd_graph")
 
     def sync(self):
         self._ck(lib().vd_sync(self.h), "vd_sync")

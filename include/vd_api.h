@@ -130,6 +130,22 @@ int vd_knn_step(vd_ctx* ctx);
 int vd_clip_step(vd_ctx* ctx);
 int vd_update_step(vd_ctx* ctx);
 int vd_lloyd(vd_ctx* ctx, int n_iter);
+/* n_iter whole iterations as replays of one HIP graph holding a complete
+ * iteration (kNN, grouping, clipping, update): on the first call one eager
+ * iteration sizes every buffer and the next one is captured.  Results are
+ * those of vd_lloyd.  The graph is dropped by vd_set_mesh, vd_set_batch,
+ * vd_ctx_set_config and by vd_set_sites with a new site count.  Replaces the
+ * host loop over iterations of compute_clipped_voro_diagram_GPU
+ * (voronoi.cu:988-1176); vd_get_timing reports only total_ms for replays. */
+int vd_lloyd_graph(vd_ctx* ctx, int n_iter);
+/* Chooses the launch parameters for the loaded problem: times `iters`
+ * iterations (after one untimed) for candidate values of site_tcap,
+ * cull_depth and grid_density, one parameter at a time, keeps the fastest
+ * configuration (written to *chosen when given) and restores the sites.
+ * These parameters change no per-pair result, so in the VD_ACCUM_F64 mode the
+ * results are bit for bit those of any other choice.  No counterpart in the
+ * reference (its launch shapes are constants, voronoi.cu:1034-1076). */
+int vd_autotune(vd_ctx* ctx, int iters, vd_config* chosen);
 int vd_sync(vd_ctx* ctx);
 /* Device times (HIP events on the context's stream) of the phases of the
  * iteration in progress, or of the last completed one. */

@@ -77,3 +77,51 @@ def test_batch_needs_one_candidate_count(vdn):
         assert e.dims()[1] == -1
         with pytest.raises(RuntimeError, match="candidate counts"):
             e.knn_step()
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_graph_replays_equal_eager_iterations(vdn, batched):
+    """Each replay of the captured iteration, with eager work and host
+    synchronisation between replays, gives the eager iteration's sites."""
+    from voronoi_amd import meshgen
+
+    probs = _instances()[:3] if batched else [meshgen.make_problem("cube", 10, 600, seed=3)]
+    cfg = dict(accum="f64", tet_k=32)
+    ref = []
+    with vdn.Engine(**cfg) as e:
+        e.set_batch(probs) if batched else (e.set_mesh(*probs[0][:2]), e.set_sites(probs[0][2]))
+        for _ in range(5):
+            e.lloyd(1)
+            ref.append(e.sites())
+    with vdn.Engine(**cfg) as g:
+        g.set_batch(probs) if batched else (g.set_mesh(*probs[0][:2]), g.set_sites(probs[0][2]))
+        g.lloyd_graph(2)  # eager sizing iteration, then the captured one
+        g.sync()
+        np.testing.assert_array_equal(g.sites(), ref[1])
+        for i in range(2, 5):
+            g.lloyd_graph(1)
+            g.sync()
+            np.testing.assert_array_equal(g.sites(), ref[i])
+            g.moments()  # eager work on the same stream between replays
+        t = g.timing()
+        assert t["total_ms"] > 0
+
+
+def test_autotune_keeps_results(vdn):
+    """The tuned launch parameters change no result (FP64 accumulation)."""
+    from voronoi_amd import meshgen
+
+    v, t, s = meshgen.make_problem("cube", 12, 1500, seed=5)
+    with vdn.Engine(accum="f64", tet_k=32) as e:
+        e.set_mesh(v, t)
+        e.set_sites(s)
+        e.lloyd(2)
+        want = e.sites()
+    with vdn.Engine(accum="f64", tet_k=32) as e:
+        e.set_mesh(v, t)
+        e.set_sites(s)
+        chosen = e.autotune(iters=1)
+        assert chosen["site_tcap"] in (0, 12, 16, 20, 24) and chosen["cull_depth"] in (8, 16, 32, 64)
+        np.testing.assert_array_equal(e.sites(), s.astype(np.float32))  # restored
+        e.lloyd(2)
+        np.testing.assert_array_equal(e.sites(), want)
