@@ -188,6 +188,23 @@ def normalize(verts: np.ndarray):
     return to_frame(verts, lo, ms)
 
 
+def batch_sizes(n_inst: int = 256, lo: int = 12, hi: int = 45, seed: int = 11) -> np.ndarray:
+    """Cube subdivisions n = round(exp(U[log lo, log hi])) of a batch of
+    instances: 6 n^3 tets each (10,368 .. 546,750 for 12 .. 45)."""
+    u = uniform24(seed, n_inst).astype(np.float64)
+    return np.rint(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.int64)
+
+
+def make_batch(n_inst: int = 256, sites_per_tet: float = 0.1, seed: int = 11):
+    """A batch of independent jittered-cube instances with sizes from
+    batch_sizes and round(sites_per_tet * tets) uniform sites each."""
+    out = []
+    for i, n in enumerate(batch_sizes(n_inst, seed=seed)):
+        n = int(n)
+        out.append(make_problem("cube", n, max(2, int(round(sites_per_tet * 6 * n ** 3))), seed=seed * 1000 + i))
+    return out
+
+
 def make_problem(kind: str, n: int, n_sites: int, seed: int = 7):
     """Normalised (verts, tets, sites) for a named construction."""
     if kind == "cube":

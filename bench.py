@@ -17,6 +17,10 @@ Reported:
                   kernel time vs the FP32 vector peak
   cpu_baseline    the oracle's FP32 restatement of the reference's kernel A
                   on the host cores, on a bounded sample of the same pairs
+  batched         a second workload, not part of `value`: 256 independent
+                  jittered-cube instances (6n^3 tets, n = round(logU[12,45]),
+                  tets/10 sites, TK = 64; 3.85e7 tets, 2.46e9 pairs) solved in
+                  one context, each iteration a replay of one HIP graph
 """
 import argparse
 import json
@@ -77,6 +81,37 @@ def run_engine(native, verts, tets, sites, steps, warmup, **cfg):
     hist = e.status_hist()
     e.close()
     return wall, tsum, hist
+
+
+def run_batch(native, steps):
+    """The batched workload: 256 independent jittered-cube instances of
+    6n^3 tets, n = round(logU[12, 45]), tets/10 uniform sites each, TK = 64,
+    solved together in one context, one Lloyd iteration per graph replay."""
+    from voronoi_amd import meshgen
+
+    t0 = time.perf_counter()
+    probs = meshgen.make_batch(256)
+    n_t = sum(len(p[1]) for p in probs)
+    n_s = sum(len(p[2]) for p in probs)
+    log(f"batch: {len(probs)} instances, {n_t} tets, {n_s} sites ({time.perf_counter() - t0:.1f}s to generate)")
+    e = native.Engine(0, site_k=K, tet_k=TK, accum="f64")
+    e.set_batch(probs)
+    del probs
+    e.lloyd_graph(2)  # eager sizing iteration + capture + one replay
+    e.sync()
+    e.timing_reset()
+    t0 = time.perf_counter()
+    e.lloyd_graph(steps)
+    e.sync()
+    wall = time.perf_counter() - t0
+    ts = e.timing_sum()
+    e.close()
+    dev_ms = ts["total_ms"] / max(ts["iters"], 1)
+    log(f"batch: {1e3 * wall / steps:.1f} ms/iter wall, {dev_ms:.1f} ms device per replayed iteration")
+    return {"instances": 256, "n_tets": n_t, "n_sites": n_s, "pairs_per_iter": n_t * TK, "steps": steps,
+            "ms_per_iter": round(1e3 * wall / steps, 3), "device_ms_per_iter": round(dev_ms, 3),
+            "iters_per_s": round(steps / wall, 4), "pairs_per_s": round(n_t * TK * steps / wall, 1),
+            "launch": "one context, hipGraph replay of a whole iteration", "accumulation": "fp64"}
 
 
 def cpu_baseline(verts, tets, sites, budget_s):
@@ -161,6 +196,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU work for the baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0, help="also time a device copy of this size (MiB)")
+    ap.add_argument("--batch-steps", type=int, default=3, help="graph replays of the 256-instance batch (0 skips)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -186,6 +222,7 @@ def main():
         pn = pts["iters"]
         plain = (args.plain_steps / pwall, pts["clip_ms"] / pn)
         log(f"plain: {1e3 * pwall / args.plain_steps:.2f} ms/iter wall; clip {pts['clip_ms']/pn:.2f} ms")
+    batch = run_batch(native, args.batch_steps) if args.batch_steps > 0 else None
     copy_gbs = device_copy(args.copy_mb) if args.copy_mb > 0 else None
     if copy_gbs:
         log(f"device copy of {args.copy_mb} MiB: {copy_gbs:.0f} GB/s (read + write)")
@@ -233,6 +270,7 @@ def main():
         "plain_clip_ms": None if plain is None else round(plain[1], 3),
         "speedup_vs_plain": None if plain is None else round(value / plain[0], 3),
         "device_copy_gbs": None if copy_gbs is None else round(copy_gbs, 1),
+        "batched": batch,
         "roofline": roof,
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["iters_per_s"], 5), "unit": "iters/s", "cores": cpu["cores"], "kind": "port",
