@@ -23,6 +23,7 @@
 #include "vd_api.h"
 #include "vd_clip.hpp"
 #include "vd_knn.hpp"
+#include "vd_internal.hpp"
 
 namespace {
 
@@ -72,7 +73,7 @@ struct vd_ctx {
   };
   std::vector<Inst> inst;
   bool batch() const { return inst.size() > 1; }
-  DevBuf<int> b_site_begin, b_cell_base, b_site_inst, b_tet_inst;
+  DevBuf<int> b_site_begin, b_cell_base, b_site_inst, b_tet_inst, b_voff;
   DevBuf<float> b_box;
   bool have_knn = false, have_clip = false;
   float mesh_lo[3] = {0, 0, 0}, mesh_hi[3] = {0, 0, 0};
@@ -314,7 +315,7 @@ void vd_ctx_destroy(vd_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->verts.release(); ctx->sites.release(); ctx->tet_xyz.release();
-  ctx->b_site_begin.release(); ctx->b_cell_base.release(); ctx->b_site_inst.release(); ctx->b_tet_inst.release();
+  ctx->b_site_begin.release(); ctx->b_voff.release(); ctx->b_cell_base.release(); ctx->b_site_inst.release(); ctx->b_tet_inst.release();
   ctx->b_box.release(); ctx->tets.release(); ctx->offsets.release();
   ctx->shape.release(); ctx->g_count.release(); ctx->g_start.release(); ctx->g_cursor.release();
   ctx->g_cell_of.release(); ctx->g_sorted_ids.release(); ctx->g_sorted_pts.release();
@@ -903,3 +904,119 @@ int vd_get_status_hist(vd_ctx* ctx, uint64_t* hist7) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------- batches from device jobs --
+
+namespace {
+// Tet vertex ids from instance-local to batch-global.
+__global__ void k_offset_tets(int4* tets, long long n, const int* tet_inst, const int* voff) {
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    const int o = voff[tet_inst[t]];
+    int4 v = tets[t];
+    v.x += o;
+    v.y += o;
+    v.z += o;
+    v.w += o;
+    tets[t] = v;
+  }
+}
+}  // namespace
+
+namespace vdi {
+
+hipStream_t stream(vd_ctx* ctx) { return ctx->stream; }
+void set_error(vd_ctx* ctx, const char* msg) { ctx->err = msg; }
+
+int assemble(vd_ctx* ctx, const JobDev* jobs, int n) {
+  if (!ctx || !jobs || n < 1) return VD_ERR_ARG;
+  VD_HIP(hipSetDevice(ctx->device), "set device");
+  ctx->drop_graph();
+  std::vector<vd_ctx::Inst> in((size_t)n);
+  long long V = 0, T = 0, S = 0;
+  for (int i = 0; i < n; ++i) {
+    in[i] = {(int)V, jobs[i].nv, (int)T, jobs[i].nt, (int)S, jobs[i].ns};
+    V += jobs[i].nv;
+    T += jobs[i].nt;
+    S += jobs[i].ns;
+  }
+  if (V > 0x7fffffffLL || T > 0x7fffffffLL || S > 0x7fffffffLL) return ctx->fail(VD_ERR_ARG, "batch too large");
+  hipStream_t st = ctx->stream;
+  VD_HIP(ctx->verts.ensure(std::max(V, 1LL)), "alloc batch");
+  VD_HIP(ctx->tets.ensure(std::max(T, 1LL)), "alloc batch");
+  VD_HIP(ctx->tet_xyz.ensure(3 * std::max(T, 1LL)), "alloc batch");
+  VD_HIP(ctx->sites.ensure(std::max(S, 1LL)), "alloc batch");
+  for (int i = 0; i < n; ++i) {
+    const JobDev& j = jobs[i];
+    const vd_ctx::Inst& q = in[i];
+    if (j.nv)
+      VD_HIP(hipMemcpyAsync(ctx->verts.p + q.v0, j.verts, sizeof(float4) * j.nv, hipMemcpyDeviceToDevice, st), "copy");
+    if (j.nt) {
+      VD_HIP(hipMemcpyAsync(ctx->tets.p + q.t0, j.tets, sizeof(int4) * j.nt, hipMemcpyDeviceToDevice, st), "copy");
+      VD_HIP(hipMemcpyAsync(ctx->tet_xyz.p + 3LL * q.t0, j.tet_xyz, 3 * sizeof(float4) * j.nt,
+                            hipMemcpyDeviceToDevice, st), "copy");
+    }
+    VD_HIP(hipMemcpyAsync(ctx->sites.p + q.s0, j.sites, sizeof(float4) * j.ns, hipMemcpyDeviceToDevice, st), "copy");
+  }
+  for (int a = 0; a < 3; ++a) {
+    ctx->mesh_lo[a] = jobs[0].box[a];
+    ctx->mesh_hi[a] = jobs[0].box[3 + a];
+    for (int i = 1; i < n; ++i) {
+      ctx->mesh_lo[a] = std::min(ctx->mesh_lo[a], jobs[i].box[a]);
+      ctx->mesh_hi[a] = std::max(ctx->mesh_hi[a], jobs[i].box[3 + a]);
+    }
+  }
+  ctx->n_verts = (int)V;
+  ctx->n_tets = (int)T;
+  ctx->n_sites = (int)S;
+  ctx->inst = in;
+  if (n > 1) {
+    std::vector<int> site_begin((size_t)n + 1), voff((size_t)n);
+    std::vector<float> box((size_t)6 * n);
+    for (int i = 0; i < n; ++i) {
+      site_begin[i] = in[i].s0;
+      voff[i] = in[i].v0;
+      for (int a = 0; a < 6; ++a) box[6 * i + a] = jobs[i].box[a];
+    }
+    site_begin[n] = (int)S;
+    VD_HIP(ctx->b_site_begin.ensure(site_begin.size()), "alloc batch");
+    VD_HIP(ctx->b_site_inst.ensure(std::max(S, 1LL)), "alloc batch");
+    VD_HIP(ctx->b_tet_inst.ensure(std::max(T, 1LL)), "alloc batch");
+    VD_HIP(ctx->b_box.ensure(box.size()), "alloc batch");
+    VD_HIP(ctx->b_voff.ensure(voff.size()), "alloc batch");
+    VD_HIP(hipMemcpyAsync(ctx->b_site_begin.p, site_begin.data(), sizeof(int) * site_begin.size(),
+                          hipMemcpyHostToDevice, st), "upload batch");
+    VD_HIP(hipMemcpyAsync(ctx->b_box.p, box.data(), sizeof(float) * box.size(), hipMemcpyHostToDevice, st),
+           "upload batch");
+    VD_HIP(hipMemcpyAsync(ctx->b_voff.p, voff.data(), sizeof(int) * voff.size(), hipMemcpyHostToDevice, st),
+           "upload batch");
+    for (int i = 0; i < n; ++i) {
+      VD_HIP(vd::fill_u32(ctx->b_site_inst.p + in[i].s0, in[i].ns, (uint32_t)i, st), "batch tables");
+      VD_HIP(vd::fill_u32(ctx->b_tet_inst.p + in[i].t0, in[i].nt, (uint32_t)i, st), "batch tables");
+    }
+    if (T > 0) {
+      hipLaunchKernelGGL(k_offset_tets, dim3((unsigned)std::min<long long>((T + 255) / 256, 8192)), dim3(256), 0, st,
+                         ctx->tets.p, T, ctx->b_tet_inst.p, ctx->b_voff.p);
+      VD_HIP(hipGetLastError(), "offset tets");
+    }
+    // the host vectors above must outlive the asynchronous uploads
+    VD_HIP(hipStreamSynchronize(st), "synchronize");
+  }
+  int rc = ensure_grid(ctx, std::max(ctx->n_sites, 1));
+  if (rc != VD_OK) return rc;
+  ctx->have_knn = ctx->have_clip = false;
+  return VD_OK;
+}
+
+int scatter_sites(vd_ctx* ctx, const JobDev* jobs, const int* idx, int n_idx) {
+  if (!ctx || (n_idx && (!jobs || !idx))) return VD_ERR_ARG;
+  for (int k = 0; k < n_idx; ++k) {
+    const int i = idx[k];
+    if (i < 0 || i >= (int)std::max<size_t>(ctx->inst.size(), 1)) return ctx->fail(VD_ERR_ARG, "bad instance");
+    const int s0 = ctx->inst.empty() ? 0 : ctx->inst[i].s0;
+    VD_HIP(hipMemcpyAsync(jobs[i].sites, ctx->sites.p + s0, sizeof(float4) * jobs[i].ns, hipMemcpyDeviceToDevice,
+                          ctx->stream), "copy sites");
+  }
+  return VD_OK;
+}
+
+}  // namespace vdi

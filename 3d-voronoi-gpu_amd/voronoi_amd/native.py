@@ -97,6 +97,16 @@ def lib():
         L.vd_kn_get_knearests.argtypes = [P, P]
         L.vd_kn_get_permutation.argtypes = [P, P]
         L.vd_compute_clipped_voro_diagram.argtypes = [P, I, P, I, P, I, I, I, I, P, ctypes.c_char_p]
+        L.vd_server_create.argtypes = [P, I, P, ctypes.c_size_t]
+        L.vd_server_destroy.argtypes = [P]
+        L.vd_server_error.argtypes = [P]
+        L.vd_server_error.restype = ctypes.c_char_p
+        L.vd_server_submit.argtypes = [P, P, I, P, I, P, I, I, P]
+        L.vd_server_step.argtypes = [P, P, P]
+        L.vd_server_job.argtypes = [P, I, P, P, P, P]
+        L.vd_server_result.argtypes = [P, I, P]
+        L.vd_server_bytes_in_use.argtypes = [P]
+        L.vd_server_bytes_in_use.restype = ctypes.c_size_t
         _lib = L
     return _lib
 
@@ -317,3 +327,66 @@ def compute_clipped_voro_diagram(verts, tets, sites, nb_iter=1, tet_k=0, out_dir
     if rc != VD_OK:
         raise RuntimeError(f"vd_compute_clipped_voro_diagram failed ({rc})")
     return s
+
+
+JOB_PENDING, JOB_RUNNING, JOB_DONE = 0, 1, 2
+
+
+class Server:
+    """Job server: continuous batching of independent Lloyd jobs on one device."""
+
+    def __init__(self, device: int = 0, budget_bytes: int = 200 << 30, **cfg):
+        self.cfg = default_config(**cfg)
+        h = ctypes.c_void_p()
+        rc = lib().vd_server_create(ctypes.byref(h), device, ctypes.byref(self.cfg), budget_bytes)
+        if rc != VD_OK:
+            raise RuntimeError(f"vd_server_create failed ({rc})")
+        self.h = h
+        self.n_sites = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vd_server_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _ck(self, rc, what):
+        if rc != VD_OK:
+            raise RuntimeError(f"{what} failed ({rc}): {lib().vd_server_error(self.h).decode()}")
+
+    def submit(self, verts, tets, sites, n_iter):
+        v = np.ascontiguousarray(verts, dtype=np.float32)
+        t = np.ascontiguousarray(tets, dtype=np.int32)
+        st = np.ascontiguousarray(sites, dtype=np.float32)
+        jid = ctypes.c_int()
+        self._ck(lib().vd_server_submit(self.h, _p(v), len(v), _p(t), len(t), _p(st), len(st), n_iter,
+                                        ctypes.byref(jid)), "vd_server_submit")
+        self.n_sites[jid.value] = len(st)
+        return jid.value
+
+    def step(self):
+        r, p = ctypes.c_int(), ctypes.c_int()
+        self._ck(lib().vd_server_step(self.h, ctypes.byref(r), ctypes.byref(p)), "vd_server_step")
+        return r.value, p.value
+
+    def job(self, jid):
+        st, it = ctypes.c_int(), ctypes.c_int()
+        w, r = ctypes.c_double(), ctypes.c_double()
+        self._ck(lib().vd_server_job(self.h, jid, ctypes.byref(st), ctypes.byref(it), ctypes.byref(w), ctypes.byref(r)),
+                 "vd_server_job")
+        return {"state": st.value, "iters": it.value, "wait_ms": w.value, "run_ms": r.value}
+
+    def result(self, jid):
+        out = np.empty((self.n_sites[jid], 3), dtype=np.float32)
+        self._ck(lib().vd_server_result(self.h, jid, _p(out)), "vd_server_result")
+        return out
+
+    def bytes_in_use(self):
+        return lib().vd_server_bytes_in_use(self.h)

@@ -204,6 +204,36 @@ int vd_kn_get_points(vd_kn* kn, float* out_xyz);
 int vd_kn_get_knearests(vd_kn* kn, unsigned int* out);
 int vd_kn_get_permutation(vd_kn* kn, unsigned int* out);
 
+/* ---- job server: continuous batching of independent jobs ---------------- */
+
+/* Jobs (mesh, sites, iteration count) wait in arrival order until their
+ * estimated device footprint fits hbm_budget_bytes, then run together: every
+ * vd_server_step runs one Lloyd iteration over all running jobs as one batch
+ * (vd_set_batch semantics, assembled on the device from the jobs' own
+ * buffers).  A job's result is the one it gets alone with the same config.
+ * cfg->tet_k must be > 0 and every job needs more than site_k sites, so that
+ * all jobs share K and TK.  No counterpart in the reference, which runs one
+ * mesh per process (test_voronoi.cu:150-229). */
+#define VD_JOB_PENDING 0
+#define VD_JOB_RUNNING 1
+#define VD_JOB_DONE 2
+typedef struct vd_server vd_server;
+int vd_server_create(vd_server** out, int device, const vd_config* cfg, size_t hbm_budget_bytes);
+void vd_server_destroy(vd_server* s);
+const char* vd_server_error(const vd_server* s);
+/* Copies the job's data; tets use 0-based ids into its own vertices. */
+int vd_server_submit(vd_server* s, const float* verts_xyz, int n_verts, const int32_t* tets, int n_tets,
+                     const float* sites_xyz, int n_sites, int n_iter, int* job_id);
+/* Admits what fits, runs one iteration of the running jobs, retires the
+ * finished ones (synchronous). */
+int vd_server_step(vd_server* s, int* n_running, int* n_pending);
+/* State (VD_JOB_*), iterations done, time from submission to admission and
+ * from admission to completion (or to now), in ms of host wall clock. */
+int vd_server_job(vd_server* s, int job_id, int* state, int* iters_done, double* wait_ms, double* run_ms);
+/* Final sites of a finished job (n_sites xyz triples). */
+int vd_server_result(vd_server* s, int job_id, float* sites_xyz);
+size_t vd_server_bytes_in_use(const vd_server* s);
+
 /* ---- whole-program entry point ------------------------------------------ */
 
 /* The reference's Lloyd driver compute_clipped_voro_diagram_GPU
