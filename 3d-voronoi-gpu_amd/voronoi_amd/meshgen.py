@@ -205,6 +205,28 @@ def make_batch(n_inst: int = 256, sites_per_tet: float = 0.1, seed: int = 11):
     return out
 
 
+def serving_jobs(n_jobs: int, rate: float, seed: int = 21, torus_frac: float = 0.05):
+    """A job stream: Poisson arrivals at `rate` jobs/s; each job a jittered
+    cube of 6n^3 tets, n = round(logU[12,45]) with tets/10 sites, or (with
+    probability torus_frac) a voxelised torus of 32..64 voxels per side with
+    tets/10 Gaussian-mixture sites; 5..50 Lloyd iterations.  Returns a list of
+    (arrival_s, (verts, tets, sites), n_iter)."""
+    u = uniform24(seed, 4 * n_jobs).astype(np.float64).reshape(n_jobs, 4)
+    sizes = batch_sizes(n_jobs, seed=seed + 1)
+    t = np.cumsum(-np.log1p(-u[:, 0]) / rate)
+    out = []
+    for i in range(n_jobs):
+        if u[i, 1] < torus_frac:
+            nv = 32 + int(u[i, 2] * 33)
+            verts, tets, _ = torus_voxels(nv)
+            prob = make_problem("torus", nv, max(100, len(tets) // 10), seed=seed * 7919 + i)
+        else:
+            n = int(sizes[i])
+            prob = make_problem("cube", n, max(100, (6 * n ** 3) // 10), seed=seed * 7919 + i)
+        out.append((float(t[i]), prob, 5 + int(u[i, 3] * 46)))
+    return out
+
+
 def make_problem(kind: str, n: int, n_sites: int, seed: int = 7):
     """Normalised (verts, tets, sites) for a named construction."""
     if kind == "cube":

@@ -21,6 +21,9 @@ Reported:
                   jittered-cube instances (6n^3 tets, n = round(logU[12,45]),
                   tets/10 sites, TK = 64; 3.85e7 tets, 2.46e9 pairs) solved in
                   one context, each iteration a replay of one HIP graph
+  served          a third workload: a Poisson stream of jobs of that size
+                  mix plus 5% torus jobs, 5..50 iterations each, through the
+                  job server (200 GB admission budget); p50/p99 job latency
 """
 import argparse
 import json
@@ -114,6 +117,47 @@ def run_batch(native, steps):
             "launch": "one context, hipGraph replay of a whole iteration", "accumulation": "fp64"}
 
 
+def run_serving(native, n_jobs, rate, budget_gb=200):
+    """Served workload: a Poisson stream of jobs (the batched workload's size
+    mix plus 5% torus jobs, 5..50 iterations each) through the job server with
+    a device-memory admission budget, in real time.  Job latency = submission
+    to completion."""
+    from voronoi_amd import meshgen
+
+    t0 = time.perf_counter()
+    jobs = meshgen.serving_jobs(n_jobs, rate)
+    log(f"serving: {n_jobs} jobs generated ({time.perf_counter() - t0:.1f}s)")
+    pair_iters = sum(len(p[1]) * TK * it for _, p, it in jobs)
+    s = native.Server(0, budget_bytes=budget_gb << 30, site_k=K, tet_k=TK, accum="f64")
+    ids = []
+    nxt = 0
+    steps = 0
+    start = time.perf_counter()
+    while True:
+        now = time.perf_counter() - start
+        while nxt < n_jobs and jobs[nxt][0] <= now:
+            ids.append(s.submit(*jobs[nxt][1], jobs[nxt][2]))
+            nxt += 1
+        running, pending = s.step()
+        steps += 1
+        if running == 0 and pending == 0:
+            if nxt == n_jobs:
+                break
+            time.sleep(max(0.0, jobs[nxt][0] - (time.perf_counter() - start)))
+    span = time.perf_counter() - start
+    lat = np.array([s.job(j)["wait_ms"] + s.job(j)["run_ms"] for j in ids])
+    wait = np.array([s.job(j)["wait_ms"] for j in ids])
+    s.close()
+    res = {"jobs": n_jobs, "arrival_rate_per_s": rate, "budget_gb": budget_gb, "steps": steps,
+           "span_s": round(span, 3), "jobs_per_s": round(n_jobs / span, 3),
+           "pairs_per_s": round(pair_iters / span, 1),
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2),
+                          "mean": round(float(lat.mean()), 2), "max": round(float(lat.max()), 2)},
+           "wait_ms": {"p50": round(float(np.percentile(wait, 50)), 2), "p99": round(float(np.percentile(wait, 99)), 2)}}
+    log(f"serving: {res}")
+    return res
+
+
 def cpu_baseline(verts, tets, sites, budget_s):
     """Oracle FP32 kernel A on a bounded, evenly spread sample of tets."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -197,6 +241,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0, help="also time a device copy of this size (MiB)")
     ap.add_argument("--batch-steps", type=int, default=3, help="graph replays of the 256-instance batch (0 skips)")
+    ap.add_argument("--serve-jobs", type=int, default=120, help="jobs of the served workload (0 skips)")
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="job arrivals per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
@@ -223,6 +269,7 @@ def main():
         plain = (args.plain_steps / pwall, pts["clip_ms"] / pn)
         log(f"plain: {1e3 * pwall / args.plain_steps:.2f} ms/iter wall; clip {pts['clip_ms']/pn:.2f} ms")
     batch = run_batch(native, args.batch_steps) if args.batch_steps > 0 else None
+    served = run_serving(native, args.serve_jobs, args.serve_rate) if args.serve_jobs > 0 else None
     copy_gbs = device_copy(args.copy_mb) if args.copy_mb > 0 else None
     if copy_gbs:
         log(f"device copy of {args.copy_mb} MiB: {copy_gbs:.0f} GB/s (read + write)")
@@ -271,6 +318,7 @@ def main():
         "speedup_vs_plain": None if plain is None else round(value / plain[0], 3),
         "device_copy_gbs": None if copy_gbs is None else round(copy_gbs, 1),
         "batched": batch,
+        "served": served,
         "roofline": roof,
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["iters_per_s"], 5), "unit": "iters/s", "cores": cpu["cores"], "kind": "port",
