@@ -4,8 +4,10 @@
 // and maps it into [0,1000]^3, reads the sites (already in that frame), runs
 // nb_iter Lloyd iterations with k candidate sites per tet (0 = heuristic), and
 // writes out.xyz, iter_<i>.xyz and record.csv in the working directory.
-// Environment: VD_ACCUM=f64 selects the FP64-accumulated form,
-// VD_MOMENTS=robust the well-conditioned moment decomposition.
+// Environment: VD_FORM=site|tet selects the tet -> cell formulation (default
+// "cells", the reference's cell -> tet source of the new sites), VD_ACCUM=f64
+// the FP64-accumulated form, VD_MOMENTS=robust the well-conditioned moment
+// decomposition.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>

@@ -72,6 +72,32 @@ struct PlaneStoreShared {
   }
 };
 
+// Kernel C: the stored cell's planes (ids 0..nvc-1) in a table shared by the
+// wave, the tet's face planes appended after them (ids nvc..nvc+3) in
+// registers.
+struct PlaneStoreCell {
+  const float4* cpl;  // [nvc] shared
+  int nvc;
+  P4 f0, f1, f2, f3;
+  __device__ __forceinline__ P4 get(int i) const {
+    const float4 b = cpl[i < nvc ? i : 0];
+    const int j = i - nvc;
+    float x = b.x, y = b.y, z = b.z, w = b.w;
+    x = j == 3 ? f3.x : x; y = j == 3 ? f3.y : y; z = j == 3 ? f3.z : z; w = j == 3 ? f3.w : w;
+    x = j == 2 ? f2.x : x; y = j == 2 ? f2.y : y; z = j == 2 ? f2.z : z; w = j == 2 ? f2.w : w;
+    x = j == 1 ? f1.x : x; y = j == 1 ? f1.y : y; z = j == 1 ? f1.z : z; w = j == 1 ? f1.w : w;
+    x = j == 0 ? f0.x : x; y = j == 0 ? f0.y : y; z = j == 0 ? f0.z : z; w = j == 0 ? f0.w : w;
+    return {x, y, z, w};
+  }
+  __device__ __forceinline__ void put(int i, const P4& p) {
+    const int j = i - nvc;
+    if (j == 0) f0 = p;
+    if (j == 1) f1 = p;
+    if (j == 2) f2 = p;
+    if (j == 3) f3 = p;
+  }
+};
+
 // Triangle (= polytope vertex) storage policies.
 // TriPlain: the vertex's plane triple only (the reference's layout); the
 //   security-radius test recomputes every vertex.
@@ -115,12 +141,16 @@ struct Cell {
   int nv, nt, nr;
   int first;
   int status;
+  bool keep_srnr;  // kernel C: a cell flagged security_radius_not_reached keeps clipping
   P4 seed;
   // cached form: slots whose vertex is not yet known to lie within the near
   // ball of the current bisector (see clip), and the d2z of the last clip
   uint32_t far;
   float last_d2z;
 
+  __device__ __forceinline__ bool ok() const {
+    return status == kSuccess || (keep_srnr && status == kSecurityRadiusNotReached);
+  }
   __device__ __forceinline__ uint32_t tri(int t) const { return ts.get(t); }
   __device__ __forceinline__ int nxt(int p) const { return nx[p * 64 + lane]; }
   __device__ __forceinline__ void set_nxt(int p, int v) { nx[p * 64 + lane] = (uint8_t)v; }
@@ -147,6 +177,7 @@ struct Cell {
   __device__ void init_tet(const P4& s, const P4 v[4]) {
     seed = s;
     status = kSuccess;
+    keep_srnr = false;
     ps.put(0, tri2plane(v[2], v[1], v[3]));
     ps.put(1, tri2plane(v[0], v[2], v[3]));
     ps.put(2, tri2plane(v[1], v[0], v[3]));
@@ -159,6 +190,34 @@ struct Cell {
     nt = 4;
     first = kNoPlane;
     far = 0xFu;
+    last_d2z = 0.f;
+  }
+
+  // Box start of kernel B: planes x >= -0.1, x <= 1000.1, y.., z.. and the
+  // 8 corner triangles (voronoi.cu:125-160).
+  __device__ void init_box(const P4& s) {
+    seed = s;
+    status = kSuccess;
+    keep_srnr = false;
+    const float lo = -0.1f, hi = 1000.f + 0.1f;
+    ps.put(0, {1.f, 0.f, 0.f, -lo});
+    ps.put(1, {-1.f, 0.f, 0.f, hi});
+    ps.put(2, {0.f, 1.f, 0.f, -lo});
+    ps.put(3, {0.f, -1.f, 0.f, hi});
+    ps.put(4, {0.f, 0.f, 1.f, -lo});
+    ps.put(5, {0.f, 0.f, -1.f, hi});
+    nv = 6;
+    put_tri(0, pack3(2, 5, 0));
+    put_tri(1, pack3(5, 3, 0));
+    put_tri(2, pack3(1, 5, 2));
+    put_tri(3, pack3(5, 1, 3));
+    put_tri(4, pack3(4, 2, 0));
+    put_tri(5, pack3(4, 0, 3));
+    put_tri(6, pack3(2, 4, 1));
+    put_tri(7, pack3(4, 3, 1));
+    nt = 8;
+    first = kNoPlane;
+    far = 0xFFu;
     last_d2z = 0.f;
   }
 
@@ -360,7 +419,7 @@ struct Cell {
         return -1;
       }
       cavity_boundary();
-      if (status != kSuccess) return -1;
+      if (!ok()) return -1;
       if (first == kNoPlane) return -1;
       int c = first;
       int guard = 0;
@@ -933,6 +992,224 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   }
 }
 
+// ----------------------------------------------------- cell -> tet form ----
+//
+// Kernel B (voronoi.cu:540-575, 648-660): every site's Voronoi cell, the box
+// [-0.1, 1000.1]^3 clipped by the bisectors of the site's neighbours with the
+// security-radius exit; one lane per site; the cell (planes, triangles) is
+// stored for kernel C with the squared radius of its farthest vertex.
+// Kernel C (voronoi.cu:620-645, 703-726): for every (tet, site) pair, the
+// stored cell clipped by the tet's four face planes, then volume and moments
+// as in kernel A.  Site-major: a wavefront takes a site, loads its cell into
+// LDS and takes its pairs 64 at a time.
+
+template <int U>
+__global__ __launch_bounds__(64) void k_cells(ClipArgs a, CellStore cs) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const int lane = threadIdx.x;
+  Cell<PlaneStoreLocal, TriPlain> c;
+  c.ps.pl = (float4*)smem;
+  c.ps.lane = lane;
+  c.ts.tr = (uint32_t*)(c.ps.pl + (size_t)a.pcap * 64);
+  c.ts.lane = lane;
+  c.nx = (uint8_t*)(c.ts.tr + (size_t)a.tcap * 64);
+  c.lane = lane;
+  c.pcap = a.pcap;
+  c.tcap = a.tcap;
+  const int K = a.K;
+  for (int base = blockIdx.x * 64; base < a.n_sites; base += gridDim.x * 64) {
+    const int s = base + lane;
+    if (s >= a.n_sites) continue;
+    const P4 sp = site_at(a, s);
+    c.init_box(sp);
+    const int32_t* nbr = a.site_knn + (size_t)s * K;
+    bool stopped = false;
+    for (int v = 0; v < K; ++v) {
+      const int z = nbr[v];
+      if ((unsigned)z >= (unsigned)a.n_sites) {  // defensive: never read outside the site array
+        c.status = kInconsistentBoundary;
+        break;
+      }
+      const P4 zp = site_at(a, z);
+      const P4 eq = bisector(sp, zp);
+      const P4 d = sub3(zp, sp);
+      const float d2z = dot3(d, d);
+      const int pid = c.nv;
+      if (pid < a.pcap) c.ps.put(pid, eq);
+      const int sec = c.clip(pid, eq, d2z);
+      if (sec >= 0 ? sec != 0 : c.security_reached(d2z)) {
+        stopped = true;
+        break;
+      }
+      if (c.status != kSuccess) break;
+    }
+    if (!stopped && c.status == kSuccess) c.status = kSecurityRadiusNotReached;
+    // store: header, planes, triangles, farthest vertex
+    float r2 = 0.f;
+    for (int t = 0; t < c.nt; ++t) r2 = fmaxf(r2, c.seed_d2(c.vertex(c.tri(t))));
+    cs.status[s] = c.status;
+    cs.nv[s] = min(c.nv, a.pcap);
+    cs.nt[s] = c.nt;
+    cs.r2[s] = r2;
+    float4* pl = cs.planes + (size_t)s * a.pcap;
+    for (int i = 0; i < min(c.nv, a.pcap); ++i) pl[i] = c.ps.pl[i * 64 + lane];
+    uint32_t* tr = cs.tris + (size_t)s * a.tcap;
+    for (int t = 0; t < c.nt; ++t) tr[t] = c.tri(t);
+  }
+}
+
+template <int MF, int ACC>
+__global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const int lane = threadIdx.x;
+  float4* cpl = (float4*)smem;                              // [pcap]
+  uint32_t* ctr = (uint32_t*)(cpl + a.pcap);                // [tcap]
+  uint32_t* tr = ctr + a.tcap;                              // [tcap][64]
+  uint8_t* nx = (uint8_t*)(tr + (size_t)a.tcap * 64);       // [pcap][64]
+  int* surv = (int*)(nx + (size_t)a.pcap * 64);             // [128] (offset is a multiple of 64)
+  Cell<PlaneStoreCell, TriPlain> c;
+  c.ps.cpl = cpl;
+  c.ts.tr = tr;
+  c.ts.lane = lane;
+  c.nx = nx;
+  c.lane = lane;
+  c.pcap = a.pcap;
+  c.tcap = a.tcap;
+  for (int s = blockIdx.x; s < a.n_sites; s += gridDim.x) {
+    const unsigned pb = a.pair_start[s], pe = a.pair_start[s + 1];
+    if (pb == pe) continue;
+    const int cst = cs.status[s];
+    const int cnv = cs.nv[s], cnt = cs.nt[s];
+    const float cr2 = cs.r2[s];
+    const P4 sp = site_at(a, s);
+    if (cst != kSuccess && cst != kSecurityRadiusNotReached) {
+      // the cell could not be built: every pair of the site takes its status
+      for (unsigned i = pb + lane; i < pe; i += 64) {
+        const unsigned pair = a.pair_ids[i];
+        if (a.pair_status) a.pair_status[pair] = (uint8_t)cst;
+        if (a.pair_vals) {
+          double* pv = a.pair_vals + 4 * (size_t)pair;
+          pv[0] = pv[1] = pv[2] = pv[3] = 0.0;
+        }
+      }
+      count_status(a, cst, false);
+      if (a.status_hist && lane == 0) atomicAdd(a.status_hist + cst, (unsigned long long)(pe - pb));
+      continue;
+    }
+    __syncthreads();
+    for (int i = lane; i < cnv; i += 64) cpl[i] = cs.planes[(size_t)s * a.pcap + i];
+    for (int t = lane; t < cnt; t += 64) ctr[t] = cs.tris[(size_t)s * a.tcap + t];
+    __syncthreads();
+    c.ps.nvc = cnv;
+    // the cell lies in the ball of radius sqrt(cr2) around the site; a tet
+    // whose bounding ball around its centroid misses that ball by more than
+    // the margin has an empty intersection with the cell
+    const float cr = sqrtf(cr2) * (1.f + 1e-5f) + kCullMargin;
+
+    auto clip_batch = [&](unsigned pair, bool live) {
+      int status = kNoIntersection;
+      if (live) {
+        P4 v[4];
+        tet_corners(a, (int)(pair / (unsigned)a.TK), v);
+        c.seed = sp;
+        c.status = cst;
+        c.keep_srnr = true;
+        c.nv = cnv;
+        c.nt = cnt;
+        c.first = kNoPlane;
+        for (int t = 0; t < cnt; ++t) c.ts.put(t, ctr[t], 0.f);
+        static constexpr int face[4][3] = {{2, 1, 3}, {0, 2, 3}, {1, 0, 3}, {0, 1, 2}};
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          const P4 eq = tri2plane(v[face[f][0]], v[face[f][1]], v[face[f][2]]);
+          const int pid = c.nv;
+          if (pid < a.pcap) c.ps.put(pid, eq);
+          (void)c.clip(pid, eq, 0.f);
+          if (c.status == kNoIntersection) break;
+        }
+        PairSums<ACC> S;
+        bool acc = false;
+        if (c.status != kNoIntersection) {
+          if (MF == 0)
+            c.template moments_ref<ACC>(S);
+          else
+            c.template moments_robust<ACC>(S);
+          if (!(isfinite((double)S.v) && isfinite((double)S.x) && isfinite((double)S.y) && isfinite((double)S.z)))
+            c.status = kNeedsExactPredicates;
+          else if (fabs((double)S.v) < 0.1)
+            c.status = kNoIntersection;
+          else
+            acc = true;
+        }
+        status = c.status;
+        if (acc) accumulate<ACC>(a, s, S);
+        if (a.pair_status) a.pair_status[pair] = (uint8_t)status;
+        if (a.pair_vals) {
+          double* pv = a.pair_vals + 4 * (size_t)pair;
+          pv[0] = acc ? (double)S.v : 0.0;
+          pv[1] = acc ? (double)S.x : 0.0;
+          pv[2] = acc ? (double)S.y : 0.0;
+          pv[3] = acc ? (double)S.z : 0.0;
+        }
+      }
+      count_status(a, status, live);
+    };
+
+    int queued = 0;
+    for (unsigned base = pb; base < pe; base += 64) {
+      const unsigned i = base + lane;
+      const bool live = i < pe;
+      const unsigned pair = live ? a.pair_ids[i] : 0u;
+      bool culled = false;
+      if (live && a.cull_depth > 0) {
+        P4 v[4];
+        tet_corners(a, (int)(pair / (unsigned)a.TK), v);
+        P4 g = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          g.x += 0.25f * v[j].x;
+          g.y += 0.25f * v[j].y;
+          g.z += 0.25f * v[j].z;
+        }
+        float tr2 = 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const P4 d = sub3(v[j], g);
+          tr2 = fmaxf(tr2, dot3(d, d));
+        }
+        const P4 d = sub3(g, sp);
+        const float gap = sqrtf(dot3(d, d)) - sqrtf(tr2) * (1.f + 1e-5f);
+        culled = gap > cr;
+        if (culled && a.pair_status) a.pair_status[pair] = (uint8_t)kNoIntersection;
+        if (culled && a.pair_vals) {
+          double* pv = a.pair_vals + 4 * (size_t)pair;
+          pv[0] = pv[1] = pv[2] = pv[3] = 0.0;
+        }
+      }
+      count_status(a, kNoIntersection, culled);
+      const bool keep = live && !culled;
+      const unsigned long long mask = __ballot(keep);
+      const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+      if (keep) surv[queued + below] = (int)pair;
+      queued += __popcll(mask);
+      __syncthreads();
+      if (queued >= 64) {
+        const unsigned mine = (unsigned)surv[lane];
+        const int rest = lane + 64 < queued ? surv[lane + 64] : 0;
+        __syncthreads();
+        if (lane + 64 < queued) surv[lane] = rest;
+        queued -= 64;
+        __syncthreads();
+        clip_batch(mine, true);
+      }
+    }
+    if (queued > 0) {
+      const unsigned mine = lane < queued ? (unsigned)surv[lane] : 0u;
+      clip_batch(mine, lane < queued);
+    }
+  }
+}
+
 // Pairs grouped by seed: a counting sort over sites.  A block takes a run of
 // kGroupPairs consecutive pairs (neighbouring tets share most candidates) and
 // counts them per site in an LDS hash table, then adds each site's count to
@@ -1118,6 +1395,32 @@ hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, i
   if (moments == 0 && acc_mode == 1) return launch_site(k_clip_site<0, 1>, n_blocks, lds, st, a);
   if (moments == 1 && acc_mode == 0) return launch_site(k_clip_site<1, 0>, n_blocks, lds, st, a);
   return launch_site(k_clip_site<1, 1>, n_blocks, lds, st, a);
+}
+
+size_t cells_lds(int pcap, int tcap) { return (size_t)64 * (16 * pcap + 4 * tcap + pcap); }
+size_t cells_clip_lds(int pcap, int tcap) {
+  return (size_t)16 * pcap + 4 * (size_t)tcap + (size_t)64 * (4 * tcap + pcap) + 4 * 128;
+}
+
+hipError_t cells_launch(const ClipArgs& a, const CellStore& cs, int moments, int acc_mode, hipStream_t st) {
+  if (a.n_sites <= 0) return hipSuccess;
+  size_t lds = cells_lds(a.pcap, a.tcap);
+  hipError_t e = hipFuncSetAttribute((const void*)k_cells<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cells<0>, dim3((unsigned)((a.n_sites + 63) / 64)), dim3(64), lds, st, a, cs);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (a.n_pairs <= 0) return hipSuccess;
+  lds = cells_clip_lds(a.pcap, a.tcap);
+  auto go = [&](auto k) -> hipError_t {
+    hipError_t e2 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e2 != hipSuccess) return e2;
+    hipLaunchKernelGGL(k, dim3((unsigned)a.n_sites), dim3(64), lds, st, a, cs);
+    return hipGetLastError();
+  };
+  if (moments == 0 && acc_mode == 0) return go(k_cells_clip<0, 0>);
+  if (moments == 0 && acc_mode == 1) return go(k_cells_clip<0, 1>);
+  if (moments == 1 && acc_mode == 0) return go(k_cells_clip<1, 0>);
+  return go(k_cells_clip<1, 1>);
 }
 
 hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count, unsigned* start,

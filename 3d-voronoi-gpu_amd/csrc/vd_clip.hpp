@@ -35,7 +35,24 @@ struct ClipArgs {
   unsigned long long* status_hist;  // optional, 7 counters
 };
 
+// Cells of kernel B, kept for kernel C: per site its status, plane and
+// triangle counts, squared radius of its farthest vertex, planes [pcap] and
+// triangles [tcap] (three plane ids in the low 24 bits).
+struct CellStore {
+  int* status;
+  int* nv;
+  int* nt;
+  float* r2;
+  float4* planes;     // n_sites x pcap
+  uint32_t* tris;     // n_sites x tcap
+};
+
 size_t clip_plain_lds(int pcap, int tcap);
+size_t cells_lds(int pcap, int tcap);
+size_t cells_clip_lds(int pcap, int tcap);
+// Kernel B over all sites, then kernel C over the pairs grouped by site
+// (a.pair_start / a.pair_ids); a.cull_depth > 0 enables the ball test.
+hipError_t cells_launch(const ClipArgs& a, const CellStore& cs, int moments, int acc_mode, hipStream_t st);
 size_t clip_site_lds(int K, int tcap);
 
 // form: 0 tet-major, 1 site-major; moments: 0 reference, 1 robust;

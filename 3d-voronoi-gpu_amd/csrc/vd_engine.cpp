@@ -104,6 +104,11 @@ struct vd_ctx {
   DevBuf<unsigned> retry_list;
   DevBuf<unsigned> retry_count;
   DevBuf<unsigned> site_counter;
+  // cells of the cell -> tet form
+  DevBuf<int> c_status, c_nv, c_nt;
+  DevBuf<float> c_r2;
+  DevBuf<float4> c_planes;
+  DevBuf<uint32_t> c_tris;
 
   // HIP events of the phases of the last kRing iterations: slot = iteration % kRing;
   // [0] start, [1] after kNN, [2] after pair grouping, [3] after clip, [4] after update,
@@ -165,7 +170,10 @@ int check_config(vd_ctx* ctx, const vd_config& c) {
   if (c.accum != VD_ACCUM_F32 && c.accum != VD_ACCUM_F64) return ctx->fail(VD_ERR_ARG, "bad accum %d", c.accum);
   if (c.moments != VD_MOMENTS_REFERENCE && c.moments != VD_MOMENTS_ROBUST)
     return ctx->fail(VD_ERR_ARG, "bad moments %d", c.moments);
-  if (c.form != VD_FORM_TET_MAJOR && c.form != VD_FORM_SITE_MAJOR) return ctx->fail(VD_ERR_ARG, "bad form %d", c.form);
+  if (c.form != VD_FORM_TET_MAJOR && c.form != VD_FORM_SITE_MAJOR && c.form != VD_FORM_CELLS)
+    return ctx->fail(VD_ERR_ARG, "bad form %d", c.form);
+  if (c.form == VD_FORM_CELLS && (vd::cells_lds(c.pcap, c.tcap) > 160 * 1024 || c.pcap > 250))
+    return ctx->fail(VD_ERR_ARG, "pcap/tcap too large for the cell kernels' LDS");
   if (!(c.grid_density > 0.05f && c.grid_density < 1000.f))
     return ctx->fail(VD_ERR_ARG, "grid_density out of range");
   // the tet-major kernel also serves the site-major form's overflow retries
@@ -325,6 +333,8 @@ void vd_ctx_destroy(vd_ctx* ctx) {
   ctx->acc_f32.release(); ctx->acc_fix.release(); ctx->moments.release();
   ctx->pair_status.release(); ctx->pair_vals.release(); ctx->hist.release();
   ctx->retry_list.release(); ctx->retry_count.release(); ctx->site_counter.release();
+  ctx->c_status.release(); ctx->c_nv.release(); ctx->c_nt.release(); ctx->c_r2.release();
+  ctx->c_planes.release(); ctx->c_tris.release();
   for (auto& row : ctx->ev)
     for (auto& e : row)
       if (e) (void)hipEventDestroy(e);
@@ -595,7 +605,7 @@ int vd_clip_step(vd_ctx* ctx) {
   a.pair_vals = c.record_pairs ? ctx->pair_vals.p : nullptr;
   a.status_hist = ctx->hist.p;
   int blocks = 0;
-  if (c.form == VD_FORM_SITE_MAJOR) {
+  if (c.form == VD_FORM_SITE_MAJOR || c.form == VD_FORM_CELLS) {
     VD_HIP(ctx->p_count.ensure(n), "alloc pairs");
     VD_HIP(ctx->p_start.ensure((size_t)n + 1), "alloc pairs");
     VD_HIP(ctx->p_cursor.ensure(n), "alloc pairs");
@@ -609,6 +619,23 @@ int vd_clip_step(vd_ctx* ctx) {
     ctx->ev_pairs = true;
     a.pair_start = ctx->p_start.p;
     a.pair_ids = ctx->p_ids.p;
+    if (c.form == VD_FORM_CELLS) {
+      // kernel B (every site's cell), then kernel C (cell x tet per pair)
+      VD_HIP(ctx->c_status.ensure(n), "alloc cells");
+      VD_HIP(ctx->c_nv.ensure(n), "alloc cells");
+      VD_HIP(ctx->c_nt.ensure(n), "alloc cells");
+      VD_HIP(ctx->c_r2.ensure(n), "alloc cells");
+      VD_HIP(ctx->c_planes.ensure((size_t)n * c.pcap), "alloc cells");
+      VD_HIP(ctx->c_tris.ensure((size_t)n * c.tcap), "alloc cells");
+      vd::CellStore cs{ctx->c_status.p, ctx->c_nv.p, ctx->c_nt.p, ctx->c_r2.p, ctx->c_planes.p, ctx->c_tris.p};
+      a.cull_depth = c.cull_depth;
+      VD_HIP(vd::cells_launch(a, cs, c.moments, c.accum, ctx->stream), "cells");
+      VD_HIP(record_phase(ctx, (5)), "event");
+      VD_HIP(record_phase(ctx, (3)), "event");
+      ctx->ev_clip = true;
+      ctx->have_clip = true;
+      return VD_OK;
+    }
     blocks = c.clip_blocks;
     VD_HIP(ctx->site_counter.ensure(1), "alloc site counter");
     VD_HIP(vd::fill_u32(ctx->site_counter.p, 1, 0u, ctx->stream), "clear site counter");

@@ -17,7 +17,7 @@ HEADER_PATH = os.path.join(os.path.dirname(PKG_ROOT), "include", "vd_api.h")
 VD_OK = 0
 ACCUM = {"f32": 0, "f64": 1}
 MOMENTS = {"reference": 0, "robust": 1}
-FORM = {"tet": 0, "site": 1}
+FORM = {"tet": 0, "site": 1, "cells": 2}
 STATUS_NAMES = [
     "triangle_overflow", "vertex_overflow", "inconsistent_boundary",
     "security_radius_not_reached", "success", "needs_exact_predicates", "no_intersection",

@@ -262,6 +262,12 @@ def main():
         f"clip {ts['clip_ms']/n:.2f} (overflow {ts['overflow_ms']/n:.2f}) update {ts['update_ms']/n:.3f} ms; "
         f"status {hist}")
 
+    # the cell -> tet formulation (kernels B + C), the reference's source of the new sites
+    cwall, cts, _ = run_engine(native, verts, tets, sites, max(1, args.plain_steps), 1, form="cells", accum="f64")
+    cells = {"iters_per_s": round(max(1, args.plain_steps) / cwall, 4),
+             "clip_ms": round(cts["clip_ms"] / max(cts["iters"], 1), 3)}
+    log(f"cells form: {1e3 * cwall / max(1, args.plain_steps):.2f} ms/iter wall; clip {cells['clip_ms']:.2f} ms")
+
     plain = None
     if args.plain_steps > 0:
         pwall, pts, _ = run_engine(native, verts, tets, sites, args.plain_steps, 1, form="tet", accum="f32")
@@ -317,6 +323,7 @@ def main():
         "plain_clip_ms": None if plain is None else round(plain[1], 3),
         "speedup_vs_plain": None if plain is None else round(value / plain[0], 3),
         "device_copy_gbs": None if copy_gbs is None else round(copy_gbs, 1),
+        "cells_form": cells,
         "batched": batch,
         "served": served,
         "roofline": roof,

@@ -53,6 +53,10 @@ extern "C" {
 /* launch form of the clipping kernel (same per-pair results) */
 #define VD_FORM_TET_MAJOR 0  /* one lane per (tet, rank) pair, the reference's order */
 #define VD_FORM_SITE_MAJOR 1 /* one wavefront per site, bisector planes shared in LDS */
+#define VD_FORM_CELLS 2      /* the cell -> tet formulation: each site's whole cell from the box
+                                (kernel B, voronoi.cu:540-575), then cell x tet per pair (kernel C,
+                                voronoi.cu:620-645,703-726); the reference's source of out.xyz.
+                                cull_depth > 0 skips pairs whose tet lies outside the cell's ball */
 
 typedef struct vd_config {
   int site_k;         /* neighbours per site; reference _K_ = 90 (params.h:9) */

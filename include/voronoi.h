@@ -5,9 +5,13 @@
  * Behaviour kept: sites updated in place in the caller's layout (SoA when
  * site_is_transposed), one record.csv row and one iter_<i>.xyz file per
  * iteration in the working directory, exit(1) on failure.  Behaviour changed:
- * sites keep their input order (the reference's grid path reorders them), and
- * the per-site accumulation is chosen by VD_ACCUM (environment: "f64" for the
- * FP64 form; default the reference's FP32 atomics).
+ * sites keep their input order (the reference's grid path reorders them).
+ * Environment: VD_FORM selects the formulation ("cells", the default, is the
+ * reference's source of the new sites: each site's whole cell clipped by the
+ * tets, voronoi.cu:1090-1109; "site" / "tet" the faster tet -> cell
+ * formulation of voronoi.cu:662-701 in its two launch forms); VD_ACCUM "f64"
+ * the FP64 accumulation (default the reference's FP32 atomics); VD_MOMENTS
+ * "robust" the well-conditioned moment decomposition.
  */
 #ifndef H_VORONOI_H
 #define H_VORONOI_H
@@ -27,6 +31,10 @@ inline void compute_clipped_voro_diagram_GPU(const std::vector<float>& vertices,
                                              int nb_Lloyd_iter = 1, int preferred_tet_k = 0) {
   vd_config cfg;
   vd_config_default(&cfg);
+  cfg.form = VD_FORM_CELLS;
+  const char* form = getenv("VD_FORM");
+  if (form && !strcmp(form, "site")) cfg.form = VD_FORM_SITE_MAJOR;
+  if (form && !strcmp(form, "tet")) cfg.form = VD_FORM_TET_MAJOR;
   const char* acc = getenv("VD_ACCUM");
   if (acc && !strcmp(acc, "f64")) cfg.accum = VD_ACCUM_F64;
   const char* mom = getenv("VD_MOMENTS");

@@ -35,7 +35,7 @@ def test_cli_on_the_reference_fixture(vdn, tmp_path):
     # the same two iterations through the C ABI
     verts, tets = fileio.read_tet(os.path.join(GOLDEN, "joint.tet"))
     sites = fileio.read_xyz(os.path.join(GOLDEN, "joint.xyz"))
-    with vdn.Engine(accum="f64") as e:
+    with vdn.Engine(accum="f64", form="cells") as e:
         e.set_mesh(meshgen.normalize(verts), tets)
         e.set_sites(sites)
         e.lloyd(2)

@@ -164,3 +164,30 @@ def test_full_size_config2_properties(vdn, O):
     pidx = (tsel[:, None] * 64 + np.arange(64)[None, :]).ravel()
     np.testing.assert_array_equal(st[pidx].astype(np.int32), r["pair_status"])
     np.testing.assert_array_equal(vals[pidx], r["pair_vals"])
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("accum", ["f32", "f64"])
+@pytest.mark.parametrize("moments", ["reference", "robust"])
+@pytest.mark.parametrize("cull", [0, 32])
+def test_cells_form_bit_exact(vdn, O, case, accum, moments, cull):
+    """The cell -> tet formulation (kernels B and C) against the oracle's,
+    pair by pair; with the ball test on, skipped pairs must be exactly those
+    the oracle finds empty."""
+    kind, n, ns = case
+    verts, tets, sites = problem(kind, n, ns)
+    sk, tk = oracle_knn(kind, n, ns, 90, 64)
+    print(f"case={case} accum={accum} moments={moments} cull={cull}", flush=True)
+    g = _run(vdn, verts, tets, sites, tet_k=64, form="cells", accum=accum, moments=moments, cull_depth=cull)
+    site_mode = O.SITE_F32 if accum == "f32" else O.SITE_FIXED
+    r = O.clip_cells(sites, sk, verts, tets, tk, geom="f32", pair_acc=accum, site_mode=site_mode, moments=moments)
+    bad = np.nonzero(g["status"].astype(np.int32) != r["pair_status"])[0]
+    assert bad.size == 0, f"{bad.size} pair statuses differ, first {bad[:5]}: {g['status'][bad[:5]]} vs {r['pair_status'][bad[:5]]}"
+    dv = np.nonzero(np.any(g["vals"] != r["pair_vals"], axis=1))[0]
+    assert dv.size == 0, f"{dv.size} pair values differ, first {dv[:5]}"
+    hist = np.bincount(r["pair_status"], minlength=7)
+    assert [g["hist"][k] for k in vdn.STATUS_NAMES] == hist.tolist()
+    if accum == "f64":
+        np.testing.assert_array_equal(g["moments"], r["site_vals"])
+    else:
+        np.testing.assert_allclose(g["moments"], r["site_vals"], rtol=1e-5, atol=1e-3)
