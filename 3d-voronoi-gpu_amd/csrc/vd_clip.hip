@@ -142,6 +142,7 @@ struct Cell {
   int first;
   int status;
   bool keep_srnr;  // kernel C: a cell flagged security_radius_not_reached keeps clipping
+  int filter;      // arithmetic filter of the conflict test: 0 off, 1 the reference's, 2 FP64 re-check
   P4 seed;
   // cached form: slots whose vertex is not yet known to lie within the near
   // ball of the current bisector (see clip), and the d2z of the last clip
@@ -152,6 +153,30 @@ struct Cell {
     return status == kSuccess || (keep_srnr && status == kSecurityRadiusNotReached);
   }
   __device__ __forceinline__ uint32_t tri(int t) const { return ts.get(t); }
+
+  // Conflict of vertex tt with plane e: det4x4 > 0 (voronoi.cu:274-306).  With
+  // the filter on, a det below the reference's FP32 error bound flags
+  // needs_exact_predicates (mode 1), or is re-evaluated in FP64 from the same
+  // planes, whose sign decides and whose own bound flags (mode 2).
+  __device__ __forceinline__ bool conflict(uint32_t tt, const P4& e) {
+    const P4 p1 = ps.get(tri_a(tt)), p2 = ps.get(tri_b(tt)), p3 = ps.get(tri_c(tt));
+    const float det = conflict_det(vertex_minors(p1, p2, p3), e);
+    if (filter) {
+      const float mx = fmaxf(fmaxf(fabsf(p1.x), fabsf(p2.x)), fmaxf(fabsf(p3.x), fabsf(e.x)));
+      const float my = fmaxf(fmaxf(fabsf(p1.y), fabsf(p2.y)), fmaxf(fabsf(p3.y), fabsf(e.y)));
+      const float mz = fmaxf(fmaxf(fabsf(p1.z), fabsf(p2.z)), fmaxf(fabsf(p3.z), fabsf(e.z)));
+      if (fabsf(det) < filter_bound32(mx, my, mz)) {
+        if (filter == 1) {
+          status = kNeedsExactPredicates;
+        } else {
+          const double d64 = conflict_det64(p1, p2, p3, e);
+          if (fabs(d64) < filter_bound64(mx, my, mz)) status = kNeedsExactPredicates;
+          return d64 > 0.0;
+        }
+      }
+    }
+    return det > 0.f;
+  }
   __device__ __forceinline__ int nxt(int p) const { return nx[p * 64 + lane]; }
   __device__ __forceinline__ void set_nxt(int p, int v) { nx[p * 64 + lane] = (uint8_t)v; }
 
@@ -322,7 +347,8 @@ struct Cell {
       // the reference's fmaxf skips); only far vertices can block.
       if (!(d2z >= last_d2z)) far = (1u << nt) - 1u;
       last_d2z = d2z;
-      const float near2 = 0.2025f * d2z;
+      // (with the filter on every determinant is evaluated, as in the reference)
+      const float near2 = filter ? -1.f : 0.2025f * d2z;
       uint32_t cand = 0, block = 0;
       for (uint32_t m = far; m; m &= m - 1) {
         const int t = __builtin_ctz(m);
@@ -339,11 +365,28 @@ struct Cell {
       uint32_t conf = 0;
       for (uint32_t m = cand; m; m &= m - 1) {
         const int t = __builtin_ctz(m);
-        const uint32_t tt = tri(t);
-        const Minors mn = vertex_minors(ps.get(tri_a(tt)), ps.get(tri_b(tt)), ps.get(tri_c(tt)));
-        if (conflict_det(mn, eqn) > 0.f) conf |= 1u << t;
+        if (conflict(tri(t), eqn)) conf |= 1u << t;
       }
       const bool kept_secure = d2z > 0.f && (block & ~conf) == 0u;
+      if (status == kNeedsExactPredicates) {
+        // the reference stops right after its conflict loop (voronoi.cu:459-461):
+        // the conflicting vertices are gone, no plane is dropped, no fan
+        while (conf) {
+          const int i = __builtin_ctz(conf);
+          --nt;
+          ++nr;
+          const uint32_t bi = 1u << i, bn = 1u << nt;
+          if (i != nt) {
+            ts.swap(i, nt);
+            conf = (conf & ~(bi | bn)) | ((conf & bn) ? bi : 0u);
+            far = (far & ~(bi | bn)) | ((far & bn) ? bi : 0u);
+          } else {
+            conf &= ~bi;
+            far &= ~bi;
+          }
+        }
+        return -1;
+      }
       if (conf == 0u) {
         --nv;
         return kept_secure ? 1 : 0;
@@ -400,9 +443,7 @@ struct Cell {
     } else {
       int i = 0;
       while (i < nt) {
-        const uint32_t t = tri(i);
-        const Minors m = vertex_minors(ps.get(tri_a(t)), ps.get(tri_b(t)), ps.get(tri_c(t)));
-        if (conflict_det(m, eqn) > 0.f) {
+        if (conflict(tri(i), eqn)) {
           --nt;
           ts.swap(i, nt);
           ++nr;
@@ -410,6 +451,7 @@ struct Cell {
           ++i;
         }
       }
+      if (status == kNeedsExactPredicates) return -1;  // voronoi.cu:459-461
       if (nr == 0) {
         --nv;
         return -1;
@@ -612,6 +654,7 @@ __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
   c.lane = lane;
   c.pcap = a.pcap;
   c.tcap = a.tcap;
+  c.filter = a.filter;
   // Either all pairs (one per lane) or the pairs of a retry list (grid-strided).
   const long long n_items = a.pair_list ? (long long)min(*a.pair_list_count, (unsigned)a.n_pairs) : a.n_pairs;
   for (long long base = (long long)blockIdx.x * 64; base < n_items; base += (long long)gridDim.x * 64) {
@@ -633,6 +676,11 @@ __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
     const P4 s = site_at(a, seed_id);
     c.init_tet(s, v);
     const int32_t* nbr = a.site_knn + (size_t)seed_id * a.K;
+    ClipArgs ak = a;
+    if (a.knn2) {
+      nbr = a.knn2 + (size_t)a.knn2_row[seed_id] * a.K2;
+      ak.K = a.K2;
+    }
     auto plane_of = [&](int v, int z, int& pid, P4& eq, float& d2z) {
       const P4 zp = site_at(a, z);
       eq = bisector(s, zp);
@@ -641,16 +689,23 @@ __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
       pid = c.nv;  // next free position
       if (pid < a.pcap) c.ps.put(pid, eq);
     };
-    acc = run_pair<MF, ACC>(a, c, nbr, plane_of, S);
+    acc = run_pair<MF, ACC>(ak, c, nbr, plane_of, S);
     status = c.status;
-    if (acc) accumulate<ACC>(a, seed_id, S);
-    if (a.pair_status) a.pair_status[pair] = (uint8_t)status;
-    if (a.pair_vals) {
-      double* pv = a.pair_vals + 4 * pair;
-      pv[0] = acc ? (double)S.v : 0.0;
-      pv[1] = acc ? (double)S.x : 0.0;
-      pv[2] = acc ? (double)S.y : 0.0;
-      pv[3] = acc ? (double)S.z : 0.0;
+    if (status == kSecurityRadiusNotReached && a.srnr_list) {
+      // clipped again with deeper neighbour lists; counted there
+      const unsigned slot = atomicAdd(a.srnr_count, 1u);
+      if (slot < (unsigned)a.n_pairs) a.srnr_list[slot] = (unsigned)pair;
+      status = -1;
+    } else {
+      if (acc) accumulate<ACC>(a, seed_id, S);
+      if (a.pair_status) a.pair_status[pair] = (uint8_t)status;
+      if (a.pair_vals) {
+        double* pv = a.pair_vals + 4 * pair;
+        pv[0] = acc ? (double)S.v : 0.0;
+        pv[1] = acc ? (double)S.x : 0.0;
+        pv[2] = acc ? (double)S.y : 0.0;
+        pv[3] = acc ? (double)S.z : 0.0;
+      }
     }
   }
   count_status(a, status, live);
@@ -726,6 +781,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   c.lane = lane;
   c.pcap = a.pcap;
   c.tcap = a.tcap;
+  c.filter = a.filter;
   if (lane < 8) hist[lane] = 0;
   const int depth = min(a.cull_depth, K);
 
@@ -917,6 +973,10 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         // recomputed from scratch at the configured caps by the retry pass
         const unsigned slot = atomicAdd(a.retry_count, 1u);
         if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = ppair;
+      } else if (status == kSecurityRadiusNotReached && a.srnr_list) {
+        // clipped again with deeper neighbour lists
+        const unsigned slot = atomicAdd(a.srnr_count, 1u);
+        if (slot < (unsigned)a.n_pairs) a.srnr_list[slot] = ppair;
       } else {
         if (acc) accumulate<ACC>(a, pslot ? site1 : site0, S);
         if (a.pair_status) a.pair_status[ppair] = (uint8_t)status;
@@ -1016,6 +1076,7 @@ __global__ __launch_bounds__(64) void k_cells(ClipArgs a, CellStore cs) {
   c.lane = lane;
   c.pcap = a.pcap;
   c.tcap = a.tcap;
+  c.filter = a.filter;
   const int K = a.K;
   for (int base = blockIdx.x * 64; base < a.n_sites; base += gridDim.x * 64) {
     const int s = base + lane;
@@ -1075,6 +1136,7 @@ __global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs) {
   c.lane = lane;
   c.pcap = a.pcap;
   c.tcap = a.tcap;
+  c.filter = a.filter;
   for (int s = blockIdx.x; s < a.n_sites; s += gridDim.x) {
     const unsigned pb = a.pair_start[s], pe = a.pair_start[s + 1];
     if (pb == pe) continue;
@@ -1312,6 +1374,21 @@ __global__ __launch_bounds__(256) void k_pairs_scatter(const int32_t* tet_knn, l
   }
 }
 
+__global__ __launch_bounds__(256) void k_srnr_rows(const unsigned* list, const unsigned* n_list, long long cap,
+                                                    const int32_t* tet_knn, int n_sites, int* row_of, int* row_site,
+                                                    unsigned* n_rows) {
+  const long long n = min((long long)*n_list, cap);
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x) {
+    const int s = tet_knn[list[j]];
+    if ((unsigned)s >= (unsigned)n_sites) continue;
+    if (atomicCAS(row_of + s, -1, -2) == -1) {
+      const unsigned r = atomicAdd(n_rows, 1u);
+      row_site[r] = s;
+      row_of[s] = (int)r;
+    }
+  }
+}
+
 // Lloyd update (voronoi.cu:728-749): site = moment / volume where volume != 0.
 __global__ void k_update_f32(float4* sites, int n, const float* acc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1395,6 +1472,13 @@ hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, i
   if (moments == 0 && acc_mode == 1) return launch_site(k_clip_site<0, 1>, n_blocks, lds, st, a);
   if (moments == 1 && acc_mode == 0) return launch_site(k_clip_site<1, 0>, n_blocks, lds, st, a);
   return launch_site(k_clip_site<1, 1>, n_blocks, lds, st, a);
+}
+
+hipError_t srnr_rows(const unsigned* list, const unsigned* n_list, long long cap, const int32_t* tet_knn, int n_sites,
+                     int* row_of, int* row_site, unsigned* n_rows, hipStream_t st) {
+  hipLaunchKernelGGL(k_srnr_rows, dim3(256), dim3(256), 0, st, list, n_list, cap, tet_knn, n_sites, row_of, row_site,
+                     n_rows);
+  return hipGetLastError();
 }
 
 size_t cells_lds(int pcap, int tcap) { return (size_t)64 * (16 * pcap + 4 * tcap + pcap); }

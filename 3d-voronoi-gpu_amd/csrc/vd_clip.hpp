@@ -25,9 +25,15 @@ struct ClipArgs {
   unsigned* retry_list;      // optional, n_pairs entries
   unsigned* retry_count;     // with retry_list
   int cull_depth;            // site-major form: bisectors tried by the emptiness test (0 = off)
+  int filter;                // arithmetic filter of the conflict test (0 off, 1 reference, 2 FP64 re-check)
   unsigned* site_counter;    // site-major form: next site to take, zero before the launch
   const unsigned* pair_list; // tet-major form: optional list of pairs to process
   const unsigned* pair_list_count;
+  unsigned* srnr_list;       // optional: pairs ending security_radius_not_reached are listed
+  unsigned* srnr_count;      //   here instead of accumulated
+  const int32_t* knn2;       // tet-major form, optional: deeper neighbour lists, K2 per row;
+  int K2;                    //   the seed s uses row knn2_row[s] instead of site_knn
+  const int* knn2_row;
   float* acc_f32;            // n_sites x 4 (vol, mx, my, mz), FP32 accumulation
   unsigned long long* acc_fix;  // n_sites x 4, 2^-20 fixed point of FP64 pair sums
   uint8_t* pair_status;      // optional, n_pairs
@@ -66,6 +72,11 @@ hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, i
 // n_sites ints, start n_sites + 1, scan_tmp scan_tmp_ints(n_sites).
 hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count, unsigned* start,
                          int* cursor, unsigned* pair_ids, unsigned* scan_tmp, hipStream_t st);
+
+// Rows of the deeper neighbour lists: every site seeding a listed pair gets a
+// row (row_site[r] = s, row_of[s] = r).  row_of must be all -1 and *n_rows 0.
+hipError_t srnr_rows(const unsigned* list, const unsigned* n_list, long long cap, const int32_t* tet_knn, int n_sites,
+                     int* row_of, int* row_site, unsigned* n_rows, hipStream_t st);
 
 hipError_t lloyd_update(float4* sites, int n, int acc_mode, const float* acc_f32, const unsigned long long* acc_fix,
                         hipStream_t st);

@@ -104,6 +104,10 @@ struct vd_ctx {
   DevBuf<unsigned> retry_list;
   DevBuf<unsigned> retry_count;
   DevBuf<unsigned> site_counter;
+  // pairs retried with deeper neighbour lists (srnr_k)
+  DevBuf<unsigned> srnr_list, srnr_count;
+  DevBuf<int> srnr_row_of, srnr_row_site;
+  DevBuf<int32_t> knn2;
   // cells of the cell -> tet form
   DevBuf<int> c_status, c_nv, c_nt;
   DevBuf<float> c_r2;
@@ -186,6 +190,9 @@ int check_config(vd_ctx* ctx, const vd_config& c) {
     return ctx->fail(VD_ERR_ARG, "cull_depth must be in [0,256], got %d", c.cull_depth);
   if (c.site_tcap != 0 && (c.site_tcap < 8 || c.site_tcap > 32))
     return ctx->fail(VD_ERR_ARG, "site_tcap must be 0 or in [8,32], got %d", c.site_tcap);
+  if (c.filter < 0 || c.filter > 2) return ctx->fail(VD_ERR_ARG, "filter must be 0, 1 or 2, got %d", c.filter);
+  if (c.srnr_k != 0 && (c.srnr_k <= c.site_k || c.srnr_k > 256))
+    return ctx->fail(VD_ERR_ARG, "srnr_k must be 0 or in (site_k, 256], got %d", c.srnr_k);
   if (c.form == VD_FORM_SITE_MAJOR && c.site_k + 4 > 254)
     return ctx->fail(VD_ERR_ARG, "site-major form needs site_k <= 250");
   return VD_OK;
@@ -278,6 +285,8 @@ void vd_config_default(vd_config* c) {
   c->clip_blocks = 0;
   c->site_tcap = 0;
   c->cull_depth = 32;
+  c->srnr_k = 0;
+  c->filter = 0;
 }
 
 int vd_ctx_create(vd_ctx** out, int device, const vd_config* cfg) {
@@ -333,6 +342,8 @@ void vd_ctx_destroy(vd_ctx* ctx) {
   ctx->acc_f32.release(); ctx->acc_fix.release(); ctx->moments.release();
   ctx->pair_status.release(); ctx->pair_vals.release(); ctx->hist.release();
   ctx->retry_list.release(); ctx->retry_count.release(); ctx->site_counter.release();
+  ctx->srnr_list.release(); ctx->srnr_count.release(); ctx->srnr_row_of.release(); ctx->srnr_row_site.release();
+  ctx->knn2.release();
   ctx->c_status.release(); ctx->c_nv.release(); ctx->c_nt.release(); ctx->c_r2.release();
   ctx->c_planes.release(); ctx->c_tris.release();
   for (auto& row : ctx->ev)
@@ -604,6 +615,7 @@ int vd_clip_step(vd_ctx* ctx) {
   a.pair_status = c.record_pairs ? ctx->pair_status.p : nullptr;
   a.pair_vals = c.record_pairs ? ctx->pair_vals.p : nullptr;
   a.status_hist = ctx->hist.p;
+  a.filter = c.filter;
   int blocks = 0;
   if (c.form == VD_FORM_SITE_MAJOR || c.form == VD_FORM_CELLS) {
     VD_HIP(ctx->p_count.ensure(n), "alloc pairs");
@@ -655,6 +667,21 @@ int vd_clip_step(vd_ctx* ctx) {
     VD_HIP(record_phase(ctx, (2)), "event");
     ctx->ev_pairs = true;
   }
+  // pairs whose cell misses the security radius: deferred, then clipped again
+  // with deeper neighbour lists (srnr_k)
+  int K2 = 0;
+  if (c.srnr_k > 0) {
+    K2 = std::min(c.srnr_k, n - 1);
+    for (const auto& q : ctx->inst) K2 = std::min(K2, q.ns - 1);
+    if (K2 <= ctx->K) K2 = 0;
+  }
+  if (K2 > 0) {
+    VD_HIP(ctx->srnr_list.ensure(std::max(n_pairs, 1LL)), "alloc srnr list");
+    VD_HIP(ctx->srnr_count.ensure(2), "alloc srnr list");
+    VD_HIP(vd::fill_u32(ctx->srnr_count.p, 2, 0u, ctx->stream), "clear srnr list");
+    a.srnr_list = ctx->srnr_list.p;
+    a.srnr_count = ctx->srnr_count.p;
+  }
   VD_HIP(vd::clip_launch(a, c.form, c.moments, c.accum, blocks, ctx->stream), "clip");
   VD_HIP(record_phase(ctx, (5)), "event");
   if (a.retry_list) {
@@ -666,6 +693,31 @@ int vd_clip_step(vd_ctx* ctx) {
     r.retry_list = nullptr;
     r.retry_count = nullptr;
     VD_HIP(vd::clip_launch(r, VD_FORM_TET_MAJOR, c.moments, c.accum, 256, ctx->stream), "clip retry");
+  }
+  if (K2 > 0) {
+    VD_HIP(ctx->srnr_row_of.ensure(n), "alloc srnr rows");
+    VD_HIP(ctx->srnr_row_site.ensure(n), "alloc srnr rows");
+    VD_HIP(ctx->knn2.ensure((size_t)n * K2), "alloc srnr rows");
+    VD_HIP(vd::fill_u32(ctx->srnr_row_of.p, n, 0xffffffffu, ctx->stream), "clear srnr rows");
+    unsigned* n_rows = ctx->srnr_count.p + 1;
+    VD_HIP(vd::srnr_rows(ctx->srnr_list.p, ctx->srnr_count.p, n_pairs, ctx->tet_knn.p, n, ctx->srnr_row_of.p,
+                         ctx->srnr_row_site.p, n_rows, ctx->stream),
+           "srnr rows");
+    VD_HIP(vd::knn_query_ids(grid_bufs(ctx), ctx->offsets.p, ctx->n_offsets, K2, ctx->sites.p, ctx->srnr_row_site.p,
+                             n_rows, n, ctx->knn2.p, ctx->stream),
+           "srnr knn");
+    vd::ClipArgs r = a;
+    r.tcap = c.tcap;
+    r.pair_list = ctx->srnr_list.p;
+    r.pair_list_count = ctx->srnr_count.p;
+    r.retry_list = nullptr;
+    r.retry_count = nullptr;
+    r.srnr_list = nullptr;
+    r.srnr_count = nullptr;
+    r.knn2 = ctx->knn2.p;
+    r.K2 = K2;
+    r.knn2_row = ctx->srnr_row_of.p;
+    VD_HIP(vd::clip_launch(r, VD_FORM_TET_MAJOR, c.moments, c.accum, 1024, ctx->stream), "clip srnr");
   }
   VD_HIP(record_phase(ctx, (3)), "event");
   ctx->ev_clip = true;

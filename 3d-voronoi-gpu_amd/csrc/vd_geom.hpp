@@ -70,6 +70,39 @@ __device__ __forceinline__ float conflict_det(const Minors& m, const P4& e) {
   return fmaf(-m.m123, e.w, fmaf(m.m124, e.z, msub(m.m234, e.x, m.m134, e.y)));
 }
 
+// The same determinant in FP64 from the FP32 planes, and the error bounds of
+// the reference's arithmetic filter (voronoi.cu:285-301 FP32, 325-341 FP64):
+// C * max|x| * max|y| * max|z| * max(max|.|)^2 over the four planes.
+__device__ __forceinline__ double msub64(double a, double b, double c, double d) { return fma(a, b, -(c * d)); }
+__device__ __forceinline__ double conflict_det64(const P4& q1, const P4& q2, const P4& q3, const P4& qe) {
+  const double p1x = q1.x, p1y = q1.y, p1z = q1.z, p1w = q1.w;
+  const double p2x = q2.x, p2y = q2.y, p2z = q2.z, p2w = q2.w;
+  const double p3x = q3.x, p3y = q3.y, p3z = q3.z, p3w = q3.w;
+  const double m12 = msub64(p1y, p2x, p1x, p2y);
+  const double m13 = msub64(p1z, p2x, p1x, p2z);
+  const double m14 = msub64(p1w, p2x, p1x, p2w);
+  const double m23 = msub64(p1z, p2y, p1y, p2z);
+  const double m24 = msub64(p1w, p2y, p1y, p2w);
+  const double m34 = msub64(p1w, p2z, p1z, p2w);
+  const double m123 = fma(m12, p3z, msub64(m23, p3x, m13, p3y));
+  const double m124 = fma(m12, p3w, msub64(m24, p3x, m14, p3y));
+  const double m134 = fma(m13, p3w, msub64(m34, p3x, m14, p3z));
+  const double m234 = fma(m23, p3w, msub64(m34, p3y, m24, p3z));
+  return fma(-m123, (double)qe.w, fma(m124, (double)qe.z, msub64(m234, (double)qe.x, m134, (double)qe.y)));
+}
+__device__ __forceinline__ float filter_bound32(float mx, float my, float mz) {
+  float eps = (float)(6.6876506e-05 * (double)mx * (double)my * (double)mz);
+  const float mm = fmaxf(fmaxf(mx, my), mz);
+  eps *= (mm * mm);
+  return eps;
+}
+__device__ __forceinline__ double filter_bound64(double mx, double my, double mz) {
+  double eps = 1.2466136531027298e-13 * mx * my * mz;
+  const double mm = fmax(fmax(mx, my), mz);
+  eps *= (mm * mm);
+  return eps;
+}
+
 // Vertex where three planes meet (voronoi.cu:238-249).
 __device__ __forceinline__ P4 vertex3(const P4& p1, const P4& p2, const P4& p3) {
   const float X = -det3(p1.w, p1.y, p1.z, p2.w, p2.y, p2.z, p3.w, p3.y, p3.z);

@@ -212,6 +212,9 @@ struct QueryArgs {
   int mode;  // 0: queries are the reference points themselves (self excluded)
              // 1: queries are tet centroids
              // 2: queries are given points (self not excluded)
+             // 3: queries are the points qpts[q_ids[q]] with ids q_ids[q] (self excluded)
+  const int* q_ids;
+  const unsigned* n_query_dev;  // optional device count, capping n_query
   const float4* verts;
   const int4* tets;
   const float4* qpts;
@@ -304,6 +307,7 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
   uint64_t* stage = stage_all[wv];
   const int q = blockIdx.x * kQueryWaves + wv;
   if (q >= a.n_query) return;  // whole wave leaves together
+  if (a.n_query_dev && (unsigned)q >= *a.n_query_dev) return;
 
   float4 qp;
   int self = -1;
@@ -313,6 +317,10 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
     qp = a.sorted_pts[q];
     self = a.sorted_ids[q];
     out_row = self;
+    if (a.pt_inst) inst = a.pt_inst[self];
+  } else if (a.mode == 3) {
+    self = a.q_ids[q];
+    qp = a.qpts[self];
     if (a.pt_inst) inst = a.pt_inst[self];
   } else if (a.q_inst) {
     inst = a.q_inst[q];
@@ -327,7 +335,7 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
     qp.y = (((v0.y + v1.y) + v2.y) + v3.y) * 0.25f;
     qp.z = (((v0.z + v1.z) + v2.z) + v3.z) * 0.25f;
     qp.w = 0.f;
-  } else {
+  } else if (a.mode == 2) {
     qp = a.qpts[q];
   }
   const int qc[3] = {cell_coord(qp.x, g.lo[0], g.inv_h, g.dim[0]), cell_coord(qp.y, g.lo[1], g.inv_h, g.dim[1]),
@@ -649,8 +657,45 @@ hipError_t knn_query(const GridBufs& gb, int n_ref, const int4* offsets, int n_o
   a.qpts = qpts;
   a.out = out;
   a.out_pitch = out_pitch;
+  a.q_ids = nullptr;
+  a.n_query_dev = nullptr;
   if (n_query <= 0) return hipSuccess;
   const dim3 grid((n_query + kQueryWaves - 1) / kQueryWaves), block(64 * kQueryWaves);
+  if (k <= 64)
+    hipLaunchKernelGGL(k_knn_query<1>, grid, block, 0, st, a);
+  else if (k <= 128)
+    hipLaunchKernelGGL(k_knn_query<2>, grid, block, 0, st, a);
+  else
+    hipLaunchKernelGGL(k_knn_query<4>, grid, block, 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t knn_query_ids(const GridBufs& gb, const int4* offsets, int n_offsets, int k, const float4* pts,
+                         const int* ids, const unsigned* n_ids_dev, int n_ids_max, int32_t* out, hipStream_t st) {
+  if (k < 1 || k > 256) return hipErrorInvalidValue;
+  if (n_ids_max <= 0) return hipSuccess;
+  QueryArgs a;
+  a.grid = gb.shape;
+  a.cell_base = gb.cell_base;
+  a.pt_inst = gb.pt_inst;
+  a.q_inst = nullptr;
+  a.sorted_pts = gb.sorted_pts;
+  a.sorted_ids = gb.sorted_ids;
+  a.start = gb.start;
+  a.n_ref = 0;
+  a.offsets = offsets;
+  a.n_offsets = n_offsets;
+  a.k = k;
+  a.n_query = n_ids_max;
+  a.mode = 3;
+  a.verts = nullptr;
+  a.tets = nullptr;
+  a.qpts = pts;
+  a.out = out;
+  a.out_pitch = 0;
+  a.q_ids = ids;
+  a.n_query_dev = n_ids_dev;
+  const dim3 grid((n_ids_max + kQueryWaves - 1) / kQueryWaves), block(64 * kQueryWaves);
   if (k <= 64)
     hipLaunchKernelGGL(k_knn_query<1>, grid, block, 0, st, a);
   else if (k <= 128)

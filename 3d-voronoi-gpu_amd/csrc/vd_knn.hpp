@@ -75,4 +75,9 @@ hipError_t knn_query(const GridBufs& gb, int n_ref, const int4* offsets, int n_o
                      int n_query, const float4* verts, const int4* tets, const float4* qpts, const int* q_inst,
                      int32_t* out, long long out_pitch, hipStream_t st);
 
+// k nearest binned points of the points pts[ids[q]], q < min(*n_ids_dev,
+// n_ids_max), each excluding itself; row q of out (row-major, k per row).
+hipError_t knn_query_ids(const GridBufs& gb, const int4* offsets, int n_offsets, int k, const float4* pts,
+                         const int* ids, const unsigned* n_ids_dev, int n_ids_max, int32_t* out, hipStream_t st);
+
 }  // namespace vd

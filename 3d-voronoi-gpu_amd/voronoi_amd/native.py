@@ -38,6 +38,8 @@ class Config(ctypes.Structure):
         ("clip_blocks", ctypes.c_int),
         ("site_tcap", ctypes.c_int),
         ("cull_depth", ctypes.c_int),
+        ("srnr_k", ctypes.c_int),
+        ("filter", ctypes.c_int),
     ]
 
 

@@ -76,6 +76,17 @@ typedef struct vd_config {
                          seed's first cull_depth bisectors (by more than the FP32 rounding
                          bound plus 0.01) has an empty intersection and is not clipped;
                          0 = off */
+  int srnr_k;         /* tet -> cell forms: 0 = the reference's behaviour (a pair whose cell
+                         does not reach the security radius with site_k neighbours is
+                         accumulated as it is, voronoi.cu:694-700); > site_k (at most 256):
+                         such pairs are clipped again with their seed's srnr_k nearest
+                         neighbours, and that result is the one accumulated */
+  int filter;         /* arithmetic filter of the conflict test (USE_ARITHMETIC_FILTER,
+                         params.h:32, off in the reference): 0 off; 1 the reference's
+                         (voronoi.cu:285-301: a determinant below its FP32 error bound
+                         flags needs_exact_predicates); 2 such a determinant is
+                         re-evaluated in FP64 from the same planes, its sign decides and
+                         only its FP64 bound (voronoi.cu:325-341) flags the pair */
 } vd_config;
 
 typedef struct vd_timing {
@@ -85,14 +96,16 @@ typedef struct vd_timing {
   float update_ms; /* Lloyd update */
   float total_ms;  /* whole iteration */
   float overflow_ms; /* part of clip_ms: re-clipping of the pairs that overflowed the
-                        site-major launch's vertex cap, at the configured caps */
+                        site-major launch's vertex cap, at the configured caps, and of the
+                        pairs retried with srnr_k neighbours */
 } vd_timing;
 
 typedef struct vd_ctx vd_ctx;
 
 /* Defaults: site_k 90, tet_k 0, pcap 64, tcap 96, VD_ACCUM_F32,
  * VD_MOMENTS_REFERENCE, VD_FORM_SITE_MAJOR, grid_density 3.1
- * (knearests.cu:243), record_pairs 0, clip_blocks 0, site_tcap 0, cull_depth 32. */
+ * (knearests.cu:243), record_pairs 0, clip_blocks 0, site_tcap 0, cull_depth 32, srnr_k 0,
+ * filter 0. */
 void vd_config_default(vd_config* cfg);
 
 /* Context on HIP device `device`.  Replaces the device selection and buffer

@@ -50,9 +50,9 @@ def lib():
         L.vdo_tet_centroids.argtypes = [P, P, i32, P]
         L.vdo_tet_k.argtypes = [i32, i32, i32]
         L.vdo_clip_tets.argtypes = [i32, i32, P, i32, P, i32, P, P, i32, P, i32, i32, i32, P, P, P, i32, P, P,
-                                    i32, i64, i64, i32]
+                                    i32, i64, i64, i32, i32]
         L.vdo_clip_cells.argtypes = [i32, i32, P, i32, P, i32, P, P, i32, P, i32, i32, i32, P, P, P, i32, P, i32,
-                                     i32]
+                                     i32, i32]
         L.vdo_lloyd_update.argtypes = [P, i32, i32, P]
         L.vdo_det4.argtypes = [P, P, P, P, i32]
         L.vdo_det4.restype = ctypes.c_double
@@ -99,8 +99,9 @@ def tet_k(n_site, n_tet, preferred=0):
 
 
 def clip_tets(sites, sknn, verts, tets, tknn, geom="f32", pair_acc="f32", site_mode=SITE_F32,
-              pcap=64, tcap=96, pair_range=None, want_pairs=True, threads=None, moments="reference"):
-    """Kernel A over all (tet, candidate) pairs. Returns dict with
+              pcap=64, tcap=96, pair_range=None, want_pairs=True, threads=None, moments="reference", filter=0):
+    """Kernel A over all (tet, candidate) pairs (filter: arithmetic filter mode
+    of the conflict test, 0 off / 1 the reference's / 2 FP64 re-check). Returns dict with
     site_vals (n_s,4) [vol, mx, my, mz], pair_status, pair_vals, events, peak_hist."""
     sites = np.ascontiguousarray(sites, dtype=np.float32)
     sknn = np.ascontiguousarray(sknn, dtype=np.int32)
@@ -119,14 +120,14 @@ def clip_tets(sites, sknn, verts, tets, tknn, geom="f32", pair_acc="f32", site_m
     rc = lib().vdo_clip_tets(0 if geom == "f32" else 1, 0 if pair_acc == "f32" else 1, _p(sites), len(sites),
                              _p(sknn), k, _p(verts), _p(tets), len(tets), _p(tknn), tk, pcap, tcap, _p(status),
                              _p(vals), _p(site_vals), site_mode, _p(ev), _p(hist), threads or nthreads(), b, e,
-                             MOMENTS[moments])
+                             MOMENTS[moments], filter)
     assert rc == 0
     return dict(site_vals=site_vals, pair_status=status, pair_vals=vals,
                 events=dict(zip(EVENT_NAMES, ev.tolist())), peak_hist=hist)
 
 
 def clip_cells(sites, sknn, verts, tets, tknn, geom="f32", pair_acc="f32", site_mode=SITE_F32,
-               pcap=64, tcap=96, threads=None, moments="reference"):
+               pcap=64, tcap=96, threads=None, moments="reference", filter=0):
     """Kernels B then C. Returns dict like clip_tets plus cell_status."""
     sites = np.ascontiguousarray(sites, dtype=np.float32)
     sknn = np.ascontiguousarray(sknn, dtype=np.int32)
@@ -141,7 +142,7 @@ def clip_cells(sites, sknn, verts, tets, tknn, geom="f32", pair_acc="f32", site_
     rc = lib().vdo_clip_cells(0 if geom == "f32" else 1, 0 if pair_acc == "f32" else 1, _p(sites), len(sites),
                               _p(sknn), sknn.shape[1], _p(verts), _p(tets), len(tets), _p(tknn), tknn.shape[1],
                               pcap, tcap, _p(status), _p(vals), _p(site_vals), site_mode, _p(cstat),
-                              threads or nthreads(), MOMENTS[moments])
+                              threads or nthreads(), MOMENTS[moments], filter)
     assert rc == 0
     return dict(site_vals=site_vals, pair_status=status, pair_vals=vals, cell_status=cstat)
 

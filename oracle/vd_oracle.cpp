@@ -373,8 +373,9 @@ int vdo_clip_tets(int geom, int pair_acc, const float* sites, int n_sites, const
                   const float* verts, const int32_t* tets, int n_tets, const int32_t* tet_knn, int tk, int pcap,
                   int tcap, int32_t* pair_status, double* pair_vals, double* site_vals, int site_mode,
                   int64_t* events, int64_t* peak_hist, int nthreads, int64_t pair_begin, int64_t pair_end,
-                  int mform) {
+                  int mform, int filter) {
   Caps caps;
+  caps.filter = filter;
   caps.pcap = pcap;
   caps.tcap = tcap;
   if (pcap < 5 || pcap > 254 || tcap < 5 || tcap > Poly<float>::TMAX) return -1;
@@ -397,8 +398,9 @@ int vdo_clip_tets(int geom, int pair_acc, const float* sites, int n_sites, const
 int vdo_clip_cells(int geom, int pair_acc, const float* sites, int n_sites, const int32_t* site_knn, int k,
                    const float* verts, const int32_t* tets, int n_tets, const int32_t* tet_knn, int tk, int pcap,
                    int tcap, int32_t* pair_status, double* pair_vals, double* site_vals, int site_mode,
-                   int32_t* cell_status, int nthreads, int mform) {
+                   int32_t* cell_status, int nthreads, int mform, int filter) {
   Caps caps;
+  caps.filter = filter;
   caps.pcap = pcap;
   caps.tcap = tcap;
   if (pcap < 7 || pcap > 254 || tcap < 9 || tcap > Poly<float>::TMAX) return -1;

@@ -151,7 +151,28 @@ inline void tet_vol_bary(const V4<R>& A, const V4<R>& B, const V4<R>& C, const V
 struct Caps {
   int pcap = 64;  // _MAX_P_ (params.h:11)
   int tcap = 96;  // _MAX_T_ (params.h:12)
+  // arithmetic filter of the conflict test (USE_ARITHMETIC_FILTER, params.h:32,
+  // off in the reference): 0 off; 1 the reference's (flag needs_exact_predicates
+  // when |det| is below its bound, voronoi.cu:285-301); 2 re-evaluate such a det
+  // in FP64 from the same planes and flag only below the FP64 bound
+  // (voronoi.cu:325-341), taking the FP64 sign
+  int filter = 0;
 };
+
+// Error bounds of the filter: C * max|x| * max|y| * max|z| * max(max|.|)^2 over
+// the four planes' x, y, z (voronoi.cu:285-297, 325-337).
+inline float filter_bound32(float mx, float my, float mz) {
+  float eps = (float)(6.6876506e-05 * (double)mx * (double)my * (double)mz);
+  const float mm = std::max(std::max(mx, my), mz);
+  eps *= (mm * mm);
+  return eps;
+}
+inline double filter_bound64(double mx, double my, double mz) {
+  double eps = 1.2466136531027298e-13 * mx * my * mz;
+  const double mm = std::max(std::max(mx, my), mz);
+  eps *= (mm * mm);
+  return eps;
+}
 
 // Work counters for the FLOP model of the clipping kernel.
 struct Events {
@@ -257,9 +278,30 @@ struct Poly {
   }
 
   // voronoi.cu:274-306: vertex t lies strictly beyond plane e.
-  bool conflict(int t, const V4<R>& e) const {
-    const Minors<R> m = vertex_minors(pl[tr[t][0]], pl[tr[t][1]], pl[tr[t][2]]);
-    return conflict_det(m, e) > R(0);
+  bool conflict(int t, const V4<R>& e) {
+    const V4<R>& p1 = pl[tr[t][0]];
+    const V4<R>& p2 = pl[tr[t][1]];
+    const V4<R>& p3 = pl[tr[t][2]];
+    const Minors<R> m = vertex_minors(p1, p2, p3);
+    const R det = conflict_det(m, e);
+    if (caps.filter) {
+      auto mx4 = [](R a, R b, R c, R d) { return std::max(std::max(std::abs(a), std::abs(b)), std::max(std::abs(c), std::abs(d))); };
+      const R mx = mx4(p1.x, p2.x, p3.x, e.x), my = mx4(p1.y, p2.y, p3.y, e.y), mz = mx4(p1.z, p2.z, p3.z, e.z);
+      const bool fp32 = sizeof(R) == sizeof(float);
+      const double eps = fp32 ? (double)filter_bound32((float)mx, (float)my, (float)mz)
+                              : filter_bound64((double)mx, (double)my, (double)mz);
+      if (std::abs((double)det) < eps) {
+        if (caps.filter == 1 || !fp32) {
+          status = needs_exact_predicates;
+        } else {
+          auto up = [](const V4<R>& v) { return V4<double>{(double)v.x, (double)v.y, (double)v.z, (double)v.w}; };
+          const double d64 = conflict_det(vertex_minors(up(p1), up(p2), up(p3)), up(e));
+          if (std::abs(d64) < filter_bound64((double)mx, (double)my, (double)mz)) status = needs_exact_predicates;
+          return d64 > 0.0;
+        }
+      }
+    }
+    return det > R(0);
   }
 
   // Bisector of the seed and site z (voronoi.cu:365-378).
@@ -359,6 +401,7 @@ struct Poly {
         ++i;
       }
     }
+    if (status == needs_exact_predicates) return;  // voronoi.cu:459-461
     if (nr == 0) {
       --nv;
       return;

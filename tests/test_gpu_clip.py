@@ -191,3 +191,109 @@ def test_cells_form_bit_exact(vdn, O, case, accum, moments, cull):
         np.testing.assert_array_equal(g["moments"], r["site_vals"])
     else:
         np.testing.assert_allclose(g["moments"], r["site_vals"], rtol=1e-5, atol=1e-3)
+
+
+def test_full_size_config3_torus(vdn, O):
+    """820,272 torus tets / 200k Gaussian-mixture sites (50x density contrast) /
+    TK=96: exact kNN, coverage away from security-radius failures, and an exact
+    check of a sample of pairs."""
+    verts, tets, sites = problem("torus", 96, 200000)
+    assert len(tets) == 820272
+    e = vdn.Engine(tet_k=96, accum="f64", moments="robust", record_pairs=1)
+    e.set_mesh(verts, tets)
+    e.set_sites(sites)
+    e.knn_step()
+    e.clip_step()
+    e.sync()
+    t = e.timing()
+    st, vals = e.pairs()
+    mom = e.moments()
+    hist = e.status_hist()
+    sk, tk = e.site_knn(), e.tet_knn()
+    e.close()
+    print("C3 timing", t, "hist", hist, flush=True)
+    np.testing.assert_array_equal(sk, O.site_knn(sites, 90))
+    rng = np.random.default_rng(9)
+    tsel = np.sort(rng.choice(len(tets), 4000, replace=False))
+    np.testing.assert_array_equal(tk[tsel], O.tet_knn(verts, tets[tsel], sites, 96))
+    assert sum(hist.values()) == len(tets) * 96
+    assert hist["needs_exact_predicates"] == 0 and hist["inconsistent_boundary"] == 0
+    V = verts.astype(np.float64)
+    a, b, c, d = (V[tets[:, j]] for j in range(4))
+    tv = -np.einsum("ij,ij->i", np.cross(b - d, c - d), a - d) / 6
+    cover = vals[:, 0].reshape(len(tets), 96).sum(1)
+    rel = np.abs(cover - tv) / tv
+    srnr = (st.reshape(len(tets), 96) == 3).any(1)
+    print("C3 per-tet coverage rel err max", rel[~srnr].max(), "tets with srnr pairs", srnr.sum(),
+          "total", mom[:, 0].sum() / tv.sum() - 1, flush=True)
+    # with K = 90 about a sixth of the tets see a cell that misses its security
+    # radius (the reference accumulates those as they are)
+    assert srnr.sum() < 0.25 * len(tets)
+    r = O.clip_tets(sites, sk, verts, tets[tsel], tk[tsel], geom="f32", pair_acc="f64", site_mode=O.SITE_F64,
+                    moments="robust")
+    pidx = (tsel[:, None] * 96 + np.arange(96)[None, :]).ravel()
+    np.testing.assert_array_equal(st[pidx].astype(np.int32), r["pair_status"])
+    np.testing.assert_array_equal(vals[pidx], r["pair_vals"])
+    # the same with the deeper retry: cells now reach their security radius
+    e = vdn.Engine(tet_k=96, accum="f64", moments="robust", record_pairs=1, srnr_k=256)
+    e.set_mesh(verts, tets)
+    e.set_sites(sites)
+    e.knn_step()
+    e.clip_step()
+    e.sync()
+    t2 = e.timing()
+    st2, vals2 = e.pairs()
+    hist2 = e.status_hist()
+    e.close()
+    cover2 = vals2[:, 0].reshape(len(tets), 96).sum(1)
+    rel2 = np.abs(cover2 - tv) / tv
+    srnr2 = (st2.reshape(len(tets), 96) == 3).any(1)
+    print("C3 srnr_k=256 timing", t2, "hist", hist2, "coverage rel err max", rel2[~srnr2].max(),
+          "p99.9", np.percentile(rel2[~srnr2], 99.9), "tets with srnr pairs", srnr2.sum(), flush=True)
+
+
+@pytest.mark.parametrize("form", ["site", "tet"])
+def test_srnr_retry_with_deeper_lists(vdn, O, form):
+    """With site_k = 20 many cells miss the security radius; srnr_k = 64
+    re-clips exactly those pairs with 64 neighbours.  Every pair must equal the
+    oracle's: the K=20 result where that one reached the radius, else the K=64
+    result; site sums follow from the pair values."""
+    verts, tets, sites = problem("cube", 20, 2000)
+    sk20 = O.site_knn(sites, 20)
+    sk64 = O.site_knn(sites, 64)
+    tk = O.tet_knn(verts, tets, sites, 64)
+    g = _run(vdn, verts, tets, sites, tet_k=64, form=form, accum="f64", site_k=20, srnr_k=64)
+    r1 = O.clip_tets(sites, sk20, verts, tets, tk, geom="f32", pair_acc="f64", site_mode=O.SITE_FIXED)
+    r2 = O.clip_tets(sites, sk64, verts, tets, tk, geom="f32", pair_acc="f64", site_mode=O.SITE_FIXED)
+    retried = r1["pair_status"] == 3
+    assert retried.sum() > 100
+    want_st = np.where(retried, r2["pair_status"], r1["pair_status"])
+    want_v = np.where(retried[:, None], r2["pair_vals"], r1["pair_vals"])
+    np.testing.assert_array_equal(g["status"].astype(np.int32), want_st)
+    np.testing.assert_array_equal(g["vals"], want_v)
+    fix = np.rint(want_v * 2.0 ** 20).astype(np.int64)
+    sums = np.zeros((len(sites), 4), dtype=np.int64)
+    np.add.at(sums, tk.ravel(), fix)
+    np.testing.assert_array_equal(g["moments"], sums / 2.0 ** 20)
+    hist = np.bincount(want_st, minlength=7)
+    assert [g["hist"][k] for k in vdn.STATUS_NAMES] == hist.tolist()
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("form", ["site", "tet", "cells"])
+@pytest.mark.parametrize("filt", [1, 2])
+def test_arithmetic_filter_bit_exact(vdn, O, case, form, filt):
+    """The conflict test's arithmetic filter (the reference's FP32 bound, or the
+    FP64 re-check) pair by pair against the oracle.  Emptiness tests off: a
+    pair they skip is no_intersection where the reference's filter may have
+    flagged it first."""
+    kind, n, ns = case
+    verts, tets, sites = problem(kind, n, ns)
+    sk, tk = oracle_knn(kind, n, ns, 90, 64)
+    print(f"case={case} form={form} filter={filt}", flush=True)
+    g = _run(vdn, verts, tets, sites, tet_k=64, form=form, accum="f64", filter=filt, cull_depth=0)
+    clip = O.clip_cells if form == "cells" else O.clip_tets
+    r = clip(sites, sk, verts, tets, tk, geom="f32", pair_acc="f64", site_mode=O.SITE_FIXED, filter=filt)
+    np.testing.assert_array_equal(g["status"].astype(np.int32), r["pair_status"])
+    np.testing.assert_array_equal(g["vals"], r["pair_vals"])
+    np.testing.assert_array_equal(g["moments"], r["site_vals"])
