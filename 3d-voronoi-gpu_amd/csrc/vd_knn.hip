@@ -299,15 +299,28 @@ __device__ __forceinline__ float cell_bound(const GridShape& g, const float4& q,
 
 constexpr int kQueryWaves = 4;
 
+// Consecutive queries (tets of one region, or sites of one grid cell) are
+// close to each other: a wave takes a run of kQueryRun of them, and each
+// query starts from the previous one's k-th distance r and their distance d:
+// the previous query's k neighbours lie within r + d of it (with itself in
+// place of one that it excludes, mode 0 and 3), so its k-th key is below that
+// bound (rounded up by a 1e-5 margin) and candidates beyond it are never
+// staged, and the cell walk stops there.
+constexpr int kQueryRun = 8;
+
 template <int M>
 __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
   __shared__ uint64_t stage_all[kQueryWaves][128];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   uint64_t* stage = stage_all[wv];
-  const int q = blockIdx.x * kQueryWaves + wv;
-  if (q >= a.n_query) return;  // whole wave leaves together
-  if (a.n_query_dev && (unsigned)q >= *a.n_query_dev) return;
+  const int q0 = (blockIdx.x * kQueryWaves + wv) * kQueryRun;
+  int nq = a.n_query;
+  if (a.n_query_dev) nq = min(nq, (int)min(*a.n_query_dev, (unsigned)a.n_query));
+  float prev_r = -1.f;
+  float4 prev_qp = {0.f, 0.f, 0.f, 0.f};
+  int prev_inst = -1;
+  for (int q = q0; q < q0 + kQueryRun && q < nq; ++q) {  // wave-uniform
 
   float4 qp;
   int self = -1;
@@ -344,7 +357,14 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
   TopK<M> L;
 #pragma unroll
   for (int s = 0; s < M; ++s) L.s[s] = ~0ull;
-  uint64_t thr = ~0ull;
+  uint64_t bound = ~0ull;
+  if (prev_r >= 0.f && prev_inst == inst) {
+    const float dx = qp.x - prev_qp.x, dy = qp.y - prev_qp.y, dz = qp.z - prev_qp.z;
+    const float b = prev_r + sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+    const float b2 = b * b * (1.f + 1e-5f);
+    if (b2 < INFINITY) bound = ((uint64_t)__float_as_uint(b2) << 32) | 0xffffffffull;
+  }
+  uint64_t thr = bound;
   int staged = 0;
   bool done = false;
 
@@ -362,7 +382,7 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
       staged -= 64;
       wave_sync();
       topk_merge<M>(L, x, lane);
-      thr = topk_kth<M>(L, a.k);
+      thr = umin64(bound, topk_kth<M>(L, a.k));
     }
   };
 
@@ -457,6 +477,11 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
         a.out[(long long)out_row * a.k + j] = id;
     }
   }
+  const uint64_t kth = topk_kth<M>(L, a.k);
+  prev_r = kth == ~0ull ? -1.f : sqrtf(key_d2(kth));
+  prev_qp = qp;
+  prev_inst = inst;
+  }  // query loop
 }
 
 }  // namespace
@@ -660,7 +685,7 @@ hipError_t knn_query(const GridBufs& gb, int n_ref, const int4* offsets, int n_o
   a.q_ids = nullptr;
   a.n_query_dev = nullptr;
   if (n_query <= 0) return hipSuccess;
-  const dim3 grid((n_query + kQueryWaves - 1) / kQueryWaves), block(64 * kQueryWaves);
+  const dim3 grid((n_query + kQueryWaves * kQueryRun - 1) / (kQueryWaves * kQueryRun)), block(64 * kQueryWaves);
   if (k <= 64)
     hipLaunchKernelGGL(k_knn_query<1>, grid, block, 0, st, a);
   else if (k <= 128)
@@ -695,7 +720,7 @@ hipError_t knn_query_ids(const GridBufs& gb, const int4* offsets, int n_offsets,
   a.out_pitch = 0;
   a.q_ids = ids;
   a.n_query_dev = n_ids_dev;
-  const dim3 grid((n_ids_max + kQueryWaves - 1) / kQueryWaves), block(64 * kQueryWaves);
+  const dim3 grid((n_ids_max + kQueryWaves * kQueryRun - 1) / (kQueryWaves * kQueryRun)), block(64 * kQueryWaves);
   if (k <= 64)
     hipLaunchKernelGGL(k_knn_query<1>, grid, block, 0, st, a);
   else if (k <= 128)
