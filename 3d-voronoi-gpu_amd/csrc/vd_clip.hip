@@ -38,8 +38,10 @@ __device__ __forceinline__ int tri_at(uint32_t t, int k) { return (int)((t >> (8
 // Storage policies --------------------------------------------------------
 //
 // PlaneStoreLocal: planes [pcap][64] per lane, ids are positions (reference).
-// PlaneStoreShared: tet faces of the lane (ids 0..3), bisector v at id 4+v in
-//   a table shared by the wave.  A separate counter nv keeps the reference's
+// PlaneStoreShared: tet faces of the lane (ids 0..3) in LDS, bisector v at id
+//   4+v in a table shared by the wave; the id picks the address, so a fetch is
+//   one ds_read_b128 (a select among registers would cost more VALU work than
+//   the LDS it saves).  A separate counter nv keeps the reference's
 //   plane count so that vertex_overflow happens exactly where it would.
 
 struct PlaneStoreLocal {
@@ -52,24 +54,15 @@ struct PlaneStoreLocal {
 // Face planes of the lane (ids 0..3) in registers, the wave's bisector table
 // (id 4+v) in LDS: a fetch reads the table entry and selects.
 struct PlaneStoreShared {
-  P4 f0, f1, f2, f3;   // tet faces
+  float4* face;        // [4][64] per lane
   const float4* bis;   // [K] shared
   int lane;
   __device__ __forceinline__ P4 get(int i) const {
-    const float4 b = bis[i < 4 ? 0 : i - 4];
-    float x = b.x, y = b.y, z = b.z, w = b.w;
-    x = i == 3 ? f3.x : x; y = i == 3 ? f3.y : y; z = i == 3 ? f3.z : z; w = i == 3 ? f3.w : w;
-    x = i == 2 ? f2.x : x; y = i == 2 ? f2.y : y; z = i == 2 ? f2.z : z; w = i == 2 ? f2.w : w;
-    x = i == 1 ? f1.x : x; y = i == 1 ? f1.y : y; z = i == 1 ? f1.z : z; w = i == 1 ? f1.w : w;
-    x = i == 0 ? f0.x : x; y = i == 0 ? f0.y : y; z = i == 0 ? f0.z : z; w = i == 0 ? f0.w : w;
-    return {x, y, z, w};
+    const float4* p = i < 4 ? face + (i * 64 + lane) : bis + (i - 4);
+    const float4 v = *p;
+    return {v.x, v.y, v.z, v.w};
   }
-  __device__ __forceinline__ void put(int i, const P4& p) {
-    if (i == 0) f0 = p;
-    if (i == 1) f1 = p;
-    if (i == 2) f2 = p;
-    if (i == 3) f3 = p;
-  }
+  __device__ __forceinline__ void put(int i, const P4& p) { face[i * 64 + lane] = make_float4(p.x, p.y, p.z, p.w); }
 };
 
 // Kernel C: the stored cell's planes (ids 0..nvc-1) in a table shared by the
@@ -752,7 +745,7 @@ __host__ __device__ inline size_t site_lds_layout(int K, int tcap, size_t* off) 
   off[0] = o; o += 16 * 2 * K4;        // bis   float4 [2][K4]
   off[1] = o;                          // (unused)
   off[2] = o; o += 4 * 2 * K4;         // bd2   float  [2][K4]
-  off[3] = o;                          // (unused)
+  off[3] = o; o += 16 * 4 * 64;        // face float4 [4][64]
   off[4] = o; o += 8 * (size_t)tcap * 64;  // tr uint2 [tcap][64]
   off[5] = o; o += 8 * kQueue;         // queue int2 [kQueue]
   off[6] = o; o += 4 * 8;              // hist  unsigned [8]
@@ -773,6 +766,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   uint2* queue = (uint2*)(smem + off[5]);
   unsigned* hist = (unsigned*)(smem + off[6]);
   Cell<PlaneStoreShared, TriCached> c;
+  c.ps.face = (float4*)(smem + off[3]);
   c.ps.bis = bis;
   c.ps.lane = lane;
   c.ts.tr = (uint2*)(smem + off[4]);
