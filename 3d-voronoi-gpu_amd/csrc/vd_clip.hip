@@ -14,6 +14,7 @@
 // Polytope state lives in LDS, slot-major ([slot][lane]) so that lanes working
 // on the same slot hit distinct banks.
 #include <algorithm>
+#include <type_traits>
 
 #include "vd_clip.hpp"
 #include "vd_geom.hpp"
@@ -634,14 +635,18 @@ __device__ __forceinline__ bool run_pair(const ClipArgs& a, Cell<PS, TS>& c, con
 
 // ------------------------------------------------------------ tet-major ----
 
-template <int MF, int ACC>
+// CACHED: vertices stored with their distance to the seed (at most 32 per
+// polytope), as in the site-major form; used to re-clip the pairs that
+// overflow the site-major launch's smaller vertex cap.
+template <int MF, int ACC, int CACHED>
 __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
   extern __shared__ __align__(16) uint8_t smem[];
   const int lane = threadIdx.x;
-  Cell<PlaneStoreLocal, TriPlain> c;
+  using TS = typename std::conditional<CACHED != 0, TriCached, TriPlain>::type;
+  Cell<PlaneStoreLocal, TS> c;
   c.ps.pl = (float4*)smem;
   c.ps.lane = lane;
-  c.ts.tr = (uint32_t*)(c.ps.pl + (size_t)a.pcap * 64);
+  c.ts.tr = (decltype(c.ts.tr))(c.ps.pl + (size_t)a.pcap * 64);
   c.ts.lane = lane;
   c.nx = (uint8_t*)(c.ts.tr + (size_t)a.tcap * 64);
   c.lane = lane;
@@ -684,7 +689,12 @@ __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
     };
     acc = run_pair<MF, ACC>(ak, c, nbr, plane_of, S);
     status = c.status;
-    if (status == kSecurityRadiusNotReached && a.srnr_list) {
+    if (status == kTriangleOverflow && a.retry_list && a.tcap < a.tcap_full) {
+      // over this launch's vertex cap: recomputed by a launch at a wider one
+      const unsigned slot = atomicAdd(a.retry_count, 1u);
+      if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = (unsigned)pair;
+      status = -1;
+    } else if (status == kSecurityRadiusNotReached && a.srnr_list) {
       // clipped again with deeper neighbour lists; counted there
       const unsigned slot = atomicAdd(a.srnr_count, 1u);
       if (slot < (unsigned)a.n_pairs) a.srnr_list[slot] = (unsigned)pair;
@@ -1419,6 +1429,7 @@ __global__ void k_moments_out(int n, int acc_mode, const float* acc_f32, const u
 }  // namespace
 
 size_t clip_plain_lds(int pcap, int tcap) { return (size_t)64 * (16 * pcap + 4 * tcap + pcap); }
+size_t clip_plain_cached_lds(int pcap, int tcap) { return (size_t)64 * (16 * pcap + 8 * tcap + pcap); }
 size_t clip_site_lds(int K, int tcap) {
   size_t off[8];
   return site_lds_layout(K, tcap, off);
@@ -1455,10 +1466,19 @@ hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, i
   if (form == 0) {
     const size_t lds = clip_plain_lds(a.pcap, a.tcap);
     const dim3 grid(a.pair_list ? (unsigned)n_blocks : (unsigned)((a.n_pairs + 63) / 64)), block(64);
-    if (moments == 0 && acc_mode == 0) return launch_lds(k_clip_plain<0, 0>, grid, block, lds, st, a);
-    if (moments == 0 && acc_mode == 1) return launch_lds(k_clip_plain<0, 1>, grid, block, lds, st, a);
-    if (moments == 1 && acc_mode == 0) return launch_lds(k_clip_plain<1, 0>, grid, block, lds, st, a);
-    return launch_lds(k_clip_plain<1, 1>, grid, block, lds, st, a);
+    if (moments == 0 && acc_mode == 0) return launch_lds(k_clip_plain<0, 0, 0>, grid, block, lds, st, a);
+    if (moments == 0 && acc_mode == 1) return launch_lds(k_clip_plain<0, 1, 0>, grid, block, lds, st, a);
+    if (moments == 1 && acc_mode == 0) return launch_lds(k_clip_plain<1, 0, 0>, grid, block, lds, st, a);
+    return launch_lds(k_clip_plain<1, 1, 0>, grid, block, lds, st, a);
+  }
+  if (form == kFormTetCached) {
+    if (a.tcap > 32) return hipErrorInvalidValue;
+    const size_t lds = clip_plain_cached_lds(a.pcap, a.tcap);
+    const dim3 grid(a.pair_list ? (unsigned)n_blocks : (unsigned)((a.n_pairs + 63) / 64)), block(64);
+    if (moments == 0 && acc_mode == 0) return launch_lds(k_clip_plain<0, 0, 1>, grid, block, lds, st, a);
+    if (moments == 0 && acc_mode == 1) return launch_lds(k_clip_plain<0, 1, 1>, grid, block, lds, st, a);
+    if (moments == 1 && acc_mode == 0) return launch_lds(k_clip_plain<1, 0, 1>, grid, block, lds, st, a);
+    return launch_lds(k_clip_plain<1, 1, 1>, grid, block, lds, st, a);
   }
   if (!a.site_counter) return hipErrorInvalidValue;
   const size_t lds = clip_site_lds(a.K, a.tcap);

@@ -54,6 +54,10 @@ struct CellStore {
 };
 
 size_t clip_plain_lds(int pcap, int tcap);
+size_t clip_plain_cached_lds(int pcap, int tcap);
+// clip_launch form of the tet-major kernel with cached vertex distances
+// (tcap <= 32), used for the site-major form's overflow pairs.
+constexpr int kFormTetCached = 3;
 size_t cells_lds(int pcap, int tcap);
 size_t cells_clip_lds(int pcap, int tcap);
 // Kernel B over all sites, then kernel C over the pairs grouped by site

@@ -103,6 +103,7 @@ struct vd_ctx {
   DevBuf<unsigned long long> hist;
   DevBuf<unsigned> retry_list;
   DevBuf<unsigned> retry_count;
+  DevBuf<unsigned> retry_list2, retry_count2;  // overflows of the first re-clip
   DevBuf<unsigned> site_counter;
   // pairs retried with deeper neighbour lists (srnr_k)
   DevBuf<unsigned> srnr_list, srnr_count;
@@ -341,7 +342,7 @@ void vd_ctx_destroy(vd_ctx* ctx) {
   ctx->p_count.release(); ctx->p_start.release(); ctx->p_cursor.release(); ctx->p_ids.release();
   ctx->acc_f32.release(); ctx->acc_fix.release(); ctx->moments.release();
   ctx->pair_status.release(); ctx->pair_vals.release(); ctx->hist.release();
-  ctx->retry_list.release(); ctx->retry_count.release(); ctx->site_counter.release();
+  ctx->retry_list.release(); ctx->retry_count.release(); ctx->retry_list2.release(); ctx->retry_count2.release(); ctx->site_counter.release();
   ctx->srnr_list.release(); ctx->srnr_count.release(); ctx->srnr_row_of.release(); ctx->srnr_row_site.release();
   ctx->knn2.release();
   ctx->c_status.release(); ctx->c_nv.release(); ctx->c_nt.release(); ctx->c_r2.release();
@@ -685,14 +686,39 @@ int vd_clip_step(vd_ctx* ctx) {
   VD_HIP(vd::clip_launch(a, c.form, c.moments, c.accum, blocks, ctx->stream), "clip");
   VD_HIP(record_phase(ctx, (5)), "event");
   if (a.retry_list) {
-    // pairs that overflowed the site-major launch's vertex cap, at the configured caps
+    // pairs that overflowed the site-major launch's vertex cap: re-clipped by
+    // the tet-major kernel with cached vertex distances at a cap of 32, and
+    // those that overflow that too by the plain one at the configured caps
     vd::ClipArgs r = a;
-    r.tcap = c.tcap;
     r.pair_list = a.retry_list;
     r.pair_list_count = a.retry_count;
     r.retry_list = nullptr;
     r.retry_count = nullptr;
-    VD_HIP(vd::clip_launch(r, VD_FORM_TET_MAJOR, c.moments, c.accum, 256, ctx->stream), "clip retry");
+    const int mid = std::min(c.tcap, 32);
+    if (mid > a.tcap && vd::clip_plain_cached_lds(c.pcap, mid) <= 160 * 1024) {
+      r.tcap = mid;
+      r.tcap_full = c.tcap;
+      if (mid < c.tcap) {
+        VD_HIP(ctx->retry_list2.ensure(std::max(n_pairs, 1LL)), "alloc retry list");
+        VD_HIP(ctx->retry_count2.ensure(1), "alloc retry list");
+        VD_HIP(vd::fill_u32(ctx->retry_count2.p, 1, 0u, ctx->stream), "clear retry count");
+        r.retry_list = ctx->retry_list2.p;
+        r.retry_count = ctx->retry_count2.p;
+      }
+      VD_HIP(vd::clip_launch(r, vd::kFormTetCached, c.moments, c.accum, 256, ctx->stream), "clip retry");
+      if (mid < c.tcap) {
+        vd::ClipArgs r3 = a;
+        r3.tcap = c.tcap;
+        r3.pair_list = ctx->retry_list2.p;
+        r3.pair_list_count = ctx->retry_count2.p;
+        r3.retry_list = nullptr;
+        r3.retry_count = nullptr;
+        VD_HIP(vd::clip_launch(r3, VD_FORM_TET_MAJOR, c.moments, c.accum, 256, ctx->stream), "clip retry");
+      }
+    } else {
+      r.tcap = c.tcap;
+      VD_HIP(vd::clip_launch(r, VD_FORM_TET_MAJOR, c.moments, c.accum, 256, ctx->stream), "clip retry");
+    }
   }
   if (K2 > 0) {
     VD_HIP(ctx->srnr_row_of.ensure(n), "alloc srnr rows");
