@@ -92,6 +92,25 @@ struct PlaneStoreCell {
   }
 };
 
+// Volume pass of the site-major form: face planes of the lane (ids 0..3) in
+// LDS, bisector v (id 4+v) rebuilt from the seed and the neighbour's position
+// with the operations that built the clipping's table, so the same bits.
+struct PlaneStoreRebuild {
+  float4* face;          // [4][64] per lane
+  const float4* sites;
+  const int32_t* nbr;    // the seed's neighbour row
+  int n_sites;
+  P4 seed;
+  int lane;
+  __device__ __forceinline__ P4 get(int i) const {
+    if (i < 4) return ld4(face, i * 64 + lane);
+    const int z = nbr[i - 4];
+    if ((unsigned)z >= (unsigned)n_sites) return {0.f, 0.f, 0.f, 0.f};  // as the table marks it
+    return bisector(seed, ld4(sites, z));
+  }
+  __device__ __forceinline__ void put(int i, const P4& p) { face[i * 64 + lane] = make_float4(p.x, p.y, p.z, p.w); }
+};
+
 // Triangle (= polytope vertex) storage policies.
 // TriPlain: the vertex's plane triple only (the reference's layout); the
 //   security-radius test recomputes every vertex.
@@ -123,6 +142,13 @@ struct TriCached {
     tr[a * 64 + lane] = y;
     tr[b * 64 + lane] = x;
   }
+};
+
+// TriRecord: the triangles of a volume-pass record (read only).
+struct TriRecord {
+  static constexpr bool kCached = false;
+  const uint32_t* tr;
+  __device__ __forceinline__ uint32_t get(int t) const { return tr[t]; }
 };
 
 template <class PS, class TS>
@@ -192,15 +218,20 @@ struct Cell {
     }
   }
 
-  // Tet start: planes of faces cbd, acd, bad, abc (voronoi.cu:163-201).
-  __device__ void init_tet(const P4& s, const P4 v[4]) {
-    seed = s;
-    status = kSuccess;
-    keep_srnr = false;
+  // Planes of faces cbd, acd, bad, abc as ids 0..3.
+  __device__ __forceinline__ void put_tet_faces(const P4 v[4]) {
     ps.put(0, tri2plane(v[2], v[1], v[3]));
     ps.put(1, tri2plane(v[0], v[2], v[3]));
     ps.put(2, tri2plane(v[1], v[0], v[3]));
     ps.put(3, tri2plane(v[0], v[1], v[2]));
+  }
+
+  // Tet start: the faces and the tet's 4 vertices (voronoi.cu:163-201).
+  __device__ void init_tet(const P4& s, const P4 v[4]) {
+    seed = s;
+    status = kSuccess;
+    keep_srnr = false;
+    put_tet_faces(v);
     nv = 4;
     put_tri(0, pack3(1, 3, 2));
     put_tri(1, pack3(0, 1, 2));
@@ -728,14 +759,17 @@ __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
 // record (3 float4 per tet), one dependent load per pair.  A pair sees the
 // same planes in the same order as in kernel A, so its result is the same bit
 // for bit.
+//
+// The volume and moments of a finished polytope are not computed here: the
+// lane stores its triangles as a record and takes its next pair at once, and
+// k_site_moments runs the volume pass over all records with every lane busy
+// (in place, a lane would either stall the wave's clipping or wait idle for
+// others to finish).
 
 constexpr int kQueue = 128;           // ring of queued (pair, slot) entries
 constexpr float kInvalidNbr = -1.f;   // table mark of an out-of-range neighbour id
-// Lane states.  A lane whose clipping is done waits until kMomentsGroup lanes
-// (or at least as many as still clip) are done too: the volume pass then runs
-// for all of them at once instead of once per finishing lane.
-constexpr int kIdle = 0, kClipping = 1, kWaiting = 2;
-constexpr int kMomentsGroup = 24;
+constexpr int kIdle = 0, kClipping = 1;  // lane states
+constexpr int kRecChunk = 128;           // record slots a wave reserves at a time
 
 // Corners of a tet from its record (x0 y0 z0 x1 | y1 z1 x2 y2 | z2 x3 y3 z3).
 __device__ __forceinline__ void corners_of(const float4& q0, const float4& q1, const float4& q2, P4 v[4]) {
@@ -767,8 +801,12 @@ template <int MF, int ACC>
 __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   extern __shared__ __align__(16) uint8_t smem[];
   const int lane = threadIdx.x;
-  const int K = a.K;
+  // bisectors in the site tables: a pair that needs more is recomputed from
+  // scratch through the retry list (the same result, at the full depth)
+  const int K = a.site_depth > 0 ? min(a.site_depth, a.K) : a.K;
+  const bool deeper = K < a.K;
   const int K4 = (K + 3) & ~3;
+  const int depth = min(a.cull_depth, K);
   size_t off[8];
   site_lds_layout(K, a.tcap, off);
   float4* bis = (float4*)(smem + off[0]);
@@ -787,7 +825,6 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   c.tcap = a.tcap;
   c.filter = a.filter;
   if (lane < 8) hist[lane] = 0;
-  const int depth = min(a.cull_depth, K);
 
   // wave-uniform scheduling state
   int site0 = -1, site1 = -1;         // site of each table slot
@@ -801,15 +838,18 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   // lane state
   int lst = kIdle;
   int pv = 0, pslot = 0;
-  unsigned ppair = 0;
+  unsigned ppair = 0, ptet = 0;
 
-  // Prefetch of the next chunk to test: its pair ids (stage 1), then its tet
-  // records (stage 2), each issued before a trip's clipping so that the loads
-  // complete under it.
-  int pf_stage = 0;  // wave-uniform
-  unsigned pf_cursor = ~0u;
-  unsigned pf_pair = 0;
+  // Loads ahead of the emptiness test: the pair ids and tet records of the
+  // chunk at pf_cursor and the pair ids of the chunk at pn_cursor.  Testing a
+  // chunk issues the next chunk's records and the ids of the one after, so
+  // the loads complete under the tests instead of stalling each chunk.
+  unsigned pf_cursor = ~0u, pn_cursor = ~0u;  // wave-uniform
+  unsigned pf_pair = 0, pn_pair = 0;
   float4 pf_r0 = {}, pf_r1 = {}, pf_r2 = {};
+
+  // record slots reserved by this wave: [rec_next, rec_end)
+  unsigned rec_next = 0, rec_end = 0;  // wave-uniform
 
   for (;;) {
     // (1) refill the queue up to 64 entries
@@ -818,17 +858,48 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         __syncthreads();  // queue entries read by the previous trip
         const unsigned i = cursor + lane;
         const bool live = i < feed_end;
-        const bool pre = pf_stage > 0 && pf_cursor == cursor;
-        const unsigned pair = pre ? pf_pair : (live ? a.pair_ids[i] : 0u);
+        unsigned pair = 0;
+        float4 r0 = {}, r1 = {}, r2 = {};
+        if (pf_cursor == cursor) {
+          pair = pf_pair;
+          r0 = pf_r0;
+          r1 = pf_r1;
+          r2 = pf_r2;
+        } else if (live) {
+          pair = a.pair_ids[i];
+          if (depth > 0) {
+            const float4* r = a.tet_xyz + 3 * (size_t)(pair / (unsigned)a.TK);
+            r0 = r[0];
+            r1 = r[1];
+            r2 = r[2];
+          }
+        }
+        // the next chunk's records (its ids came with the previous chunk, or
+        // now) and the ids of the chunk after it
+        const unsigned c1 = cursor + 64, c2 = cursor + 128;
+        pf_cursor = ~0u;
+        if (c1 < feed_end) {
+          const bool live1 = c1 + lane < feed_end;
+          pf_pair = pn_cursor == c1 ? pn_pair : (live1 ? a.pair_ids[c1 + lane] : 0u);
+          if (depth > 0 && live1) {
+            const float4* r = a.tet_xyz + 3 * (size_t)(pf_pair / (unsigned)a.TK);
+            pf_r0 = r[0];
+            pf_r1 = r[1];
+            pf_r2 = r[2];
+          }
+          pf_cursor = c1;
+        }
+        pn_cursor = ~0u;
+        if (c2 < feed_end) {
+          pn_pair = c2 + lane < feed_end ? a.pair_ids[c2 + lane] : 0u;
+          pn_cursor = c2;
+        }
         bool culled = false;
         if (depth > 0) {
           P4 v[4] = {};
           float pmax = 0.f, vmin2 = 0.f;
           if (live) {
-            if (pre && pf_stage == 2)
-              corners_of(pf_r0, pf_r1, pf_r2, v);
-            else
-              tet_corners(a, (int)(pair / (unsigned)a.TK), v);
+            corners_of(r0, r1, r2, v);
             const P4 fs = site_at(a, feed_slot ? site1 : site0);
             vmin2 = INFINITY;
 #pragma unroll
@@ -889,7 +960,6 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         if (keep) queue[(qtail + (unsigned)below) % kQueue] = make_uint2(pair, (unsigned)feed_slot);
         qtail += (unsigned)__popcll(mask);
         cursor += 64;
-        pf_stage = 0;
         continue;
       }
       if (no_more_sites) break;
@@ -906,7 +976,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
       const unsigned pb = a.pair_start[s], pe = a.pair_start[s + 1];
       if (pb == pe) continue;
       const P4 sp = site_at(a, s);
-      const int32_t* nbr = a.site_knn + (size_t)s * K;
+      const int32_t* nbr = a.site_knn + (size_t)s * a.K;
       __syncthreads();  // the slot's last readers are done
       float4* eb = bis + other * K4;
       float* db = bd2 + other * K4;
@@ -943,32 +1013,18 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
       pslot = (int)q.y;
       pv = 0;
       lst = kClipping;
+      ptet = ppair / (unsigned)a.TK;
       P4 v[4];
-      tet_corners(a, (int)(ppair / (unsigned)a.TK), v);
+      tet_corners(a, (int)ptet, v);
       c.ps.bis = bis + pslot * K4;
       c.init_tet(site_at(a, pslot ? site1 : site0), v);
     }
     qhead += min((unsigned)__popcll(idle), avail);
     const int n_clip = __popcll(__ballot(lst == kClipping));
-    const int n_wait = __popcll(__ballot(lst == kWaiting));
-    if (n_clip + n_wait == 0) {
+    if (n_clip == 0) {
       if (no_more_sites && cursor >= feed_end && qhead == qtail) break;
       continue;
     }
-    if (pf_stage == 1 && pf_cursor == cursor) {
-      if (depth > 0 && cursor + lane < feed_end) {
-        const float4* r = a.tet_xyz + 3 * (size_t)(pf_pair / (unsigned)a.TK);
-        pf_r0 = r[0];
-        pf_r1 = r[1];
-        pf_r2 = r[2];
-      }
-      pf_stage = 2;
-    } else if (pf_stage == 0 && cursor < feed_end) {
-      pf_cursor = cursor;
-      pf_pair = cursor + lane < feed_end ? a.pair_ids[cursor + lane] : 0;
-      pf_stage = 1;
-    }
-
     // Result of a finished pair: retry entry, or accumulation and records.
     auto finish = [&](const PairSums<ACC>& S, bool acc) {
       const int status = c.status;
@@ -996,29 +1052,30 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
       lst = kIdle;
     };
 
-    if (n_wait > 0 && (n_wait >= kMomentsGroup || n_wait >= n_clip)) {
-      // (3a) volume and moments of the waiting lanes' polytopes, together
-      if (lst == kWaiting) {
-        PairSums<ACC> S;
-        if (MF == 0)
-          c.template moments_ref<ACC>(S);
-        else
-          c.template moments_robust<ACC>(S);
-        bool acc = false;
-        // vertices formed by parallel planes (degenerate input) give
-        // non-finite sums; such a pair is not accumulated
-        if (!(isfinite((double)S.v) && isfinite((double)S.x) && isfinite((double)S.y) && isfinite((double)S.z)))
-          c.status = kNeedsExactPredicates;
-        else if (fabs((double)S.v) < 0.1)
-          c.status = kNoIntersection;
-        else
-          acc = true;
-        finish(S, acc);
-      }
-    } else if (lst == kClipping) {
-      // (3b) one bisector for every clipping lane: kernel A's loop
+    // Volume and moments of this lane's polytope, in place.
+    auto moments_now = [&]() {
+      PairSums<ACC> S;
+      if (MF == 0)
+        c.template moments_ref<ACC>(S);
+      else
+        c.template moments_robust<ACC>(S);
+      bool acc = false;
+      // vertices formed by parallel planes (degenerate input) give
+      // non-finite sums; such a pair is not accumulated
+      if (!(isfinite((double)S.v) && isfinite((double)S.x) && isfinite((double)S.y) && isfinite((double)S.z)))
+        c.status = kNeedsExactPredicates;
+      else if (fabs((double)S.v) < 0.1)
+        c.status = kNoIntersection;
+      else
+        acc = true;
+      finish(S, acc);
+    };
+
+    bool defer = false;
+    if (lst == kClipping) {
+      // (3) one bisector for every clipping lane: kernel A's loop
       // (voronoi.cu:679-700) one trip at a time
-      bool done = false, fin = false;
+      bool done = false, fin = false, redo = false;
       if (K == 0) {
         c.status = kSecurityRadiusNotReached;
         done = fin = true;
@@ -1034,26 +1091,159 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
           } else if (c.status != kSuccess) {
             done = true;
           } else if (++pv == K) {
-            c.status = kSecurityRadiusNotReached;
-            done = fin = true;
+            if (deeper) {
+              redo = true;
+            } else {
+              c.status = kSecurityRadiusNotReached;
+              fin = true;
+            }
+            done = true;
           }
         }
       }
       if (done) {
-        if (fin && c.status != kNoIntersection) {
-          lst = kWaiting;
+        if (redo) {
+          const unsigned slot = atomicAdd(a.retry_count, 1u);
+          if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = ppair;
+          lst = kIdle;
+        } else if (fin && c.status != kNoIntersection) {
+          defer = true;
         } else {
           PairSums<ACC> S;
           finish(S, false);
         }
       }
     }
+    // (4) records of the polytopes for the volume pass, one slot per
+    // deferring lane, from slots the wave reserves kRecChunk at a time (one
+    // global counter shared by every wave would serialise them)
+    const unsigned long long dm = __ballot(defer);
+    if (dm != 0) {
+      unsigned slot = ~0u;
+      if (a.rec_meta) {
+        const unsigned need = (unsigned)__popcll(dm);
+        const unsigned rank =
+            (unsigned)__builtin_amdgcn_mbcnt_hi((uint32_t)(dm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)dm, 0));
+        const unsigned have = rec_end - rec_next;
+        if (have >= need) {
+          slot = rec_next + rank;
+          rec_next += need;
+        } else {
+          const int leader = __builtin_ctzll(dm);
+          unsigned base = 0;
+          if (lane == leader) base = atomicAdd(a.rec_count, (unsigned)kRecChunk);
+          base = __shfl(base, leader);
+          slot = rank < have ? rec_next + rank : base + (rank - have);
+          rec_next = base + (need - have);
+          rec_end = base + (unsigned)kRecChunk;
+        }
+      }
+      if (defer) {
+        if (a.rec_meta && slot < a.rec_cap) {
+          a.rec_meta[slot] = make_uint4(ppair, (unsigned)(pslot ? site1 : site0), ptet,
+                                        (unsigned)c.nt | ((unsigned)c.status << 8));
+          uint4* dst = (uint4*)(a.rec_tris + (size_t)slot * rec_stride(a.tcap));
+          for (int t = 0; t < c.nt; t += 4) {
+            uint4 w;
+            w.x = c.tri(t);
+            w.y = t + 1 < c.nt ? c.tri(t + 1) : 0u;
+            w.z = t + 2 < c.nt ? c.tri(t + 2) : 0u;
+            w.w = t + 3 < c.nt ? c.tri(t + 3) : 0u;
+            dst[t >> 2] = w;
+          }
+          lst = kIdle;
+        } else {
+          moments_now();  // past the record capacity
+        }
+      }
+    }
   }
+  // reserved record slots left unused are marked for the volume pass
+  if (a.rec_meta)
+    for (unsigned r = rec_next + lane; r < rec_end && r < a.rec_cap; r += 64) a.rec_meta[r] = make_uint4(~0u, 0u, 0u, 0u);
   __syncthreads();
   if (a.status_hist && lane < 7) {
     const unsigned n = hist[lane] + (lane == kNoIntersection ? n_culled : 0u);
     if (n) atomicAdd(a.status_hist + lane, (unsigned long long)n);
   }
+}
+
+// Volume pass of the site-major form: one lane per record.  The planes are
+// rebuilt from their ids (faces from the tet's corners, bisectors from the
+// seed's neighbour row), the sums run over the triangles in the order they
+// were stored, and the pair's outcome follows as in the in-place pass:
+// non-finite sums flag needs_exact_predicates, a volume below 0.1
+// no_intersection; then a vertex-cap overflow goes to the retry list and
+// security_radius_not_reached to the deeper-list re-clip when they are on,
+// anything else is accumulated.
+constexpr int kMomentsBlock = 256;
+
+template <int MF, int ACC>
+__global__ __launch_bounds__(kMomentsBlock) void k_site_moments(ClipArgs a) {
+  __shared__ float4 face_all[kMomentsBlock / 64][4][64];
+  __shared__ unsigned hist[8];
+  const int lane = threadIdx.x & 63;
+  if (threadIdx.x < 8) hist[threadIdx.x] = 0;
+  __syncthreads();
+  Cell<PlaneStoreRebuild, TriRecord> c;
+  c.ps.face = &face_all[threadIdx.x >> 6][0][0];
+  c.ps.sites = a.sites;
+  c.ps.n_sites = a.n_sites;
+  c.ps.lane = lane;
+  c.keep_srnr = false;
+  const unsigned n = min(*a.rec_count, a.rec_cap);
+  const int stride = rec_stride(a.tcap);
+  for (unsigned i = blockIdx.x * kMomentsBlock + threadIdx.x; i < n; i += gridDim.x * kMomentsBlock) {
+    const uint4 m = a.rec_meta[i];
+    if (m.x == ~0u) continue;  // a reserved slot left unused
+    const unsigned pair = m.x;
+    const int s = (int)m.y;
+    P4 v[4];
+    tet_corners(a, (int)m.z, v);
+    c.seed = site_at(a, s);
+    c.ps.seed = c.seed;
+    c.ps.nbr = a.site_knn + (size_t)s * a.K;
+    c.put_tet_faces(v);
+    c.ts.tr = a.rec_tris + (size_t)i * stride;
+    c.nt = (int)(m.w & 255u);
+    int status = (int)(m.w >> 8);
+    PairSums<ACC> S;
+    if (MF == 0)
+      c.template moments_ref<ACC>(S);
+    else
+      c.template moments_robust<ACC>(S);
+    bool acc = false;
+    if (!(isfinite((double)S.v) && isfinite((double)S.x) && isfinite((double)S.y) && isfinite((double)S.z)))
+      status = kNeedsExactPredicates;
+    else if (fabs((double)S.v) < 0.1)
+      status = kNoIntersection;
+    else
+      acc = true;
+    // then as the site-major kernel's finish step
+    if (status == kTriangleOverflow && a.retry_list && a.tcap < a.tcap_full) {
+      const unsigned slot = atomicAdd(a.retry_count, 1u);
+      if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = pair;
+      continue;
+    }
+    if (status == kSecurityRadiusNotReached && a.srnr_list) {
+      const unsigned slot = atomicAdd(a.srnr_count, 1u);
+      if (slot < (unsigned)a.n_pairs) a.srnr_list[slot] = pair;
+      continue;
+    }
+    if (acc) accumulate<ACC>(a, s, S);
+    if (a.pair_status) a.pair_status[pair] = (uint8_t)status;
+    if (a.pair_vals) {
+      double* pvals = a.pair_vals + 4 * (size_t)pair;
+      pvals[0] = acc ? (double)S.v : 0.0;
+      pvals[1] = acc ? (double)S.x : 0.0;
+      pvals[2] = acc ? (double)S.y : 0.0;
+      pvals[3] = acc ? (double)S.z : 0.0;
+    }
+    atomicAdd(&hist[status], 1u);
+  }
+  __syncthreads();
+  if (a.status_hist && threadIdx.x < 7 && hist[threadIdx.x])
+    atomicAdd(a.status_hist + threadIdx.x, (unsigned long long)hist[threadIdx.x]);
 }
 
 // ----------------------------------------------------- cell -> tet form ----
@@ -1481,11 +1671,31 @@ hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, i
     return launch_lds(k_clip_plain<1, 1, 1>, grid, block, lds, st, a);
   }
   if (!a.site_counter) return hipErrorInvalidValue;
-  const size_t lds = clip_site_lds(a.K, a.tcap);
-  if (moments == 0 && acc_mode == 0) return launch_site(k_clip_site<0, 0>, n_blocks, lds, st, a);
-  if (moments == 0 && acc_mode == 1) return launch_site(k_clip_site<0, 1>, n_blocks, lds, st, a);
-  if (moments == 1 && acc_mode == 0) return launch_site(k_clip_site<1, 0>, n_blocks, lds, st, a);
-  return launch_site(k_clip_site<1, 1>, n_blocks, lds, st, a);
+  if (a.rec_meta && (!a.rec_tris || !a.rec_count)) return hipErrorInvalidValue;
+  if (a.site_depth > 0 && a.site_depth < a.K && (!a.retry_list || !a.retry_count)) return hipErrorInvalidValue;
+  const size_t lds = clip_site_lds(a.site_depth > 0 ? std::min(a.site_depth, a.K) : a.K, a.tcap);
+  hipError_t e;
+  if (moments == 0 && acc_mode == 0)
+    e = launch_site(k_clip_site<0, 0>, n_blocks, lds, st, a);
+  else if (moments == 0 && acc_mode == 1)
+    e = launch_site(k_clip_site<0, 1>, n_blocks, lds, st, a);
+  else if (moments == 1 && acc_mode == 0)
+    e = launch_site(k_clip_site<1, 0>, n_blocks, lds, st, a);
+  else
+    e = launch_site(k_clip_site<1, 1>, n_blocks, lds, st, a);
+  if (e != hipSuccess || !a.rec_meta || a.rec_cap == 0) return e;
+  // the records' volume pass; the record count is read on the device
+  const dim3 grid((unsigned)std::min<long long>(((long long)a.rec_cap + kMomentsBlock - 1) / kMomentsBlock, 4096)),
+      block(kMomentsBlock);
+  if (moments == 0 && acc_mode == 0)
+    hipLaunchKernelGGL((k_site_moments<0, 0>), grid, block, 0, st, a);
+  else if (moments == 0 && acc_mode == 1)
+    hipLaunchKernelGGL((k_site_moments<0, 1>), grid, block, 0, st, a);
+  else if (moments == 1 && acc_mode == 0)
+    hipLaunchKernelGGL((k_site_moments<1, 0>), grid, block, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_site_moments<1, 1>), grid, block, 0, st, a);
+  return hipGetLastError();
 }
 
 hipError_t srnr_rows(const unsigned* list, const unsigned* n_list, long long cap, const int32_t* tet_knn, int n_sites,

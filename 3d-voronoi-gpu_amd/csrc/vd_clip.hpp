@@ -27,6 +27,16 @@ struct ClipArgs {
   int cull_depth;            // site-major form: bisectors tried by the emptiness test (0 = off)
   int filter;                // arithmetic filter of the conflict test (0 off, 1 reference, 2 FP64 re-check)
   unsigned* site_counter;    // site-major form: next site to take, zero before the launch
+  int site_depth;            // site-major form: bisectors per site table (0 = K); a pair
+                             // that needs more goes to retry_list
+  // site-major form, optional: polytopes whose clipping ended are handed to
+  // the volume pass as records: rec_meta (pair, seed, tet, nt | status << 8)
+  // and rec_stride(tcap) triangles each; rec_count is zero before the launch.
+  // Past rec_cap records a lane computes its volume in place.
+  uint4* rec_meta;
+  uint32_t* rec_tris;
+  unsigned* rec_count;
+  unsigned rec_cap;
   const unsigned* pair_list; // tet-major form: optional list of pairs to process
   const unsigned* pair_list_count;
   unsigned* srnr_list;       // optional: pairs ending security_radius_not_reached are listed
@@ -63,12 +73,16 @@ size_t cells_clip_lds(int pcap, int tcap);
 // Kernel B over all sites, then kernel C over the pairs grouped by site
 // (a.pair_start / a.pair_ids); a.cull_depth > 0 enables the ball test.
 hipError_t cells_launch(const ClipArgs& a, const CellStore& cs, int moments, int acc_mode, hipStream_t st);
-size_t clip_site_lds(int K, int tcap);
+// LDS of the site-major kernel with `depth` bisectors per site table.
+size_t clip_site_lds(int depth, int tcap);
+// Triangle words per record of the site-major form's volume pass.
+__host__ __device__ inline int rec_stride(int tcap) { return (tcap + 3) & ~3; }
 
 // form: 0 tet-major, 1 site-major; moments: 0 reference, 1 robust;
 // acc_mode: 0 FP32 atomics, 1 FP64 pair sums + fixed-point site sums.
 // n_blocks: workgroups of the site-major form (0 = as many as stay resident),
-// and of the tet-major form when it runs over a.pair_list.
+// and of the tet-major form when it runs over a.pair_list.  The site-major
+// form with records (a.rec_meta) also launches their volume pass.
 hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, int n_blocks, hipStream_t st);
 
 // Groups the pairs of an n_tets x TK candidate table (at most 2^32 - 1 pairs)

@@ -48,6 +48,9 @@ struct DevBuf {
   }
 };
 
+// Bisectors per site table of the site-major launch for K neighbours.
+int site_depth_of(const vd_config& c, int K) { return c.site_depth > 0 ? std::min(c.site_depth, K) : K; }
+
 int tet_k_heuristic(int n_site, int n_tet, int preferred) {
   // voronoi.cu:936
   if (preferred) return preferred;
@@ -105,6 +108,10 @@ struct vd_ctx {
   DevBuf<unsigned> retry_count;
   DevBuf<unsigned> retry_list2, retry_count2;  // overflows of the first re-clip
   DevBuf<unsigned> site_counter;
+  // site-major form: records of finished polytopes for the volume pass
+  DevBuf<uint4> rec_meta;
+  DevBuf<uint32_t> rec_tris;
+  DevBuf<unsigned> rec_count;
   // pairs retried with deeper neighbour lists (srnr_k)
   DevBuf<unsigned> srnr_list, srnr_count;
   DevBuf<int> srnr_row_of, srnr_row_site;
@@ -185,13 +192,17 @@ int check_config(vd_ctx* ctx, const vd_config& c) {
   if (vd::clip_plain_lds(c.pcap, c.tcap) > 160 * 1024)
     return ctx->fail(VD_ERR_ARG, "pcap/tcap too large for the tet-major kernel's LDS");
   const int fast_tcap = c.site_tcap > 0 ? std::min(c.site_tcap, c.tcap) : std::min(c.tcap, 16);
-  if (c.form == VD_FORM_SITE_MAJOR && vd::clip_site_lds(c.site_k, fast_tcap) > 160 * 1024)
+  if (c.form == VD_FORM_SITE_MAJOR && vd::clip_site_lds(site_depth_of(c, c.site_k), fast_tcap) > 160 * 1024)
     return ctx->fail(VD_ERR_ARG, "site_k/site_tcap too large for the site-major form's LDS");
   if (c.cull_depth < 0 || c.cull_depth > 256)
     return ctx->fail(VD_ERR_ARG, "cull_depth must be in [0,256], got %d", c.cull_depth);
   if (c.site_tcap != 0 && (c.site_tcap < 8 || c.site_tcap > 32))
     return ctx->fail(VD_ERR_ARG, "site_tcap must be 0 or in [8,32], got %d", c.site_tcap);
   if (c.filter < 0 || c.filter > 2) return ctx->fail(VD_ERR_ARG, "filter must be 0, 1 or 2, got %d", c.filter);
+  if (c.site_depth < 0 || c.site_depth > 256)
+    return ctx->fail(VD_ERR_ARG, "site_depth must be in [0,256], got %d", c.site_depth);
+  if (c.site_records < -1)
+    return ctx->fail(VD_ERR_ARG, "site_records must be -1, 0 or a record count, got %d", c.site_records);
   if (c.srnr_k != 0 && (c.srnr_k <= c.site_k || c.srnr_k > 256))
     return ctx->fail(VD_ERR_ARG, "srnr_k must be 0 or in (site_k, 256], got %d", c.srnr_k);
   if (c.form == VD_FORM_SITE_MAJOR && c.site_k + 4 > 254)
@@ -288,6 +299,8 @@ void vd_config_default(vd_config* c) {
   c->cull_depth = 32;
   c->srnr_k = 0;
   c->filter = 0;
+  c->site_records = 0;
+  c->site_depth = 0;
 }
 
 int vd_ctx_create(vd_ctx** out, int device, const vd_config* cfg) {
@@ -345,6 +358,7 @@ void vd_ctx_destroy(vd_ctx* ctx) {
   ctx->retry_list.release(); ctx->retry_count.release(); ctx->retry_list2.release(); ctx->retry_count2.release(); ctx->site_counter.release();
   ctx->srnr_list.release(); ctx->srnr_count.release(); ctx->srnr_row_of.release(); ctx->srnr_row_site.release();
   ctx->knn2.release();
+  ctx->rec_meta.release(); ctx->rec_tris.release(); ctx->rec_count.release();
   ctx->c_status.release(); ctx->c_nv.release(); ctx->c_nt.release(); ctx->c_r2.release();
   ctx->c_planes.release(); ctx->c_tris.release();
   for (auto& row : ctx->ev)
@@ -654,7 +668,8 @@ int vd_clip_step(vd_ctx* ctx) {
     VD_HIP(vd::fill_u32(ctx->site_counter.p, 1, 0u, ctx->stream), "clear site counter");
     a.site_counter = ctx->site_counter.p;
     const int fast_tcap = c.site_tcap > 0 ? std::min(c.site_tcap, c.tcap) : std::min(c.tcap, 16);
-    if (fast_tcap < c.tcap) {
+    a.site_depth = site_depth_of(c, ctx->K);
+    if (fast_tcap < c.tcap || a.site_depth < ctx->K) {
       VD_HIP(ctx->retry_list.ensure(std::max(n_pairs, 1LL)), "alloc retry list");
       VD_HIP(ctx->retry_count.ensure(1), "alloc retry list");
       VD_HIP(vd::fill_u32(ctx->retry_count.p, 1, 0u, ctx->stream), "clear retry count");
@@ -664,6 +679,19 @@ int vd_clip_step(vd_ctx* ctx) {
     a.tcap = fast_tcap;
     a.tcap_full = c.tcap;
     a.cull_depth = c.cull_depth;
+    if (c.site_records >= 0) {
+      // automatic: one pair in ten, plus the slots the waves reserve ahead
+      const long long want = c.site_records > 0 ? c.site_records : n_pairs / 10 + 262144;
+      const unsigned cap = (unsigned)std::max(1LL, std::min({want, n_pairs, (long long)UINT32_MAX}));
+      VD_HIP(ctx->rec_meta.ensure(cap), "alloc records");
+      VD_HIP(ctx->rec_tris.ensure((size_t)cap * vd::rec_stride(fast_tcap)), "alloc records");
+      VD_HIP(ctx->rec_count.ensure(1), "alloc records");
+      VD_HIP(vd::fill_u32(ctx->rec_count.p, 1, 0u, ctx->stream), "clear record count");
+      a.rec_meta = ctx->rec_meta.p;
+      a.rec_tris = ctx->rec_tris.p;
+      a.rec_count = ctx->rec_count.p;
+      a.rec_cap = cap;
+    }
   } else {
     VD_HIP(record_phase(ctx, (2)), "event");
     ctx->ev_pairs = true;

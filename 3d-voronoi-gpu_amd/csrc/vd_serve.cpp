@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "vd_api.h"
+#include "vd_clip.hpp"
 #include "vd_internal.hpp"
 
 namespace {
@@ -75,11 +76,18 @@ struct vd_server {
 
   // Device bytes of a running job: its own buffers plus its share of the
   // batch's working memory (batch copies, kNN tables, grouped pairs, retry
-  // list, grid).
+  // list, grid, and the site-major form's volume-pass records: one per ten
+  // pairs).
   size_t footprint(int nv, int nt, int ns) const {
     const size_t K = (size_t)cfg.site_k, TK = (size_t)cfg.tet_k;
     const size_t own = 16 * (size_t)nv + (16 + 48) * (size_t)nt + 16 * (size_t)ns + 4 * 256;
-    const size_t work = 16 * (size_t)nv + (16 + 48 + 12 * TK + 8) * (size_t)nt + (16 + 4 * K + 64) * (size_t)ns;
+    size_t rec = 0;
+    if (cfg.form == VD_FORM_SITE_MAJOR && cfg.site_records >= 0) {
+      const int ft = cfg.site_tcap > 0 ? std::min(cfg.site_tcap, cfg.tcap) : std::min(cfg.tcap, 16);
+      rec = (16 + 4 * (size_t)vd::rec_stride(ft)) * TK / 10;
+    }
+    const size_t work =
+        16 * (size_t)nv + (16 + 48 + 12 * TK + rec + 8) * (size_t)nt + (16 + 4 * K + 64) * (size_t)ns;
     return own + work;
   }
 

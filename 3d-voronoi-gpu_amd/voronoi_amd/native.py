@@ -40,6 +40,8 @@ class Config(ctypes.Structure):
         ("cull_depth", ctypes.c_int),
         ("srnr_k", ctypes.c_int),
         ("filter", ctypes.c_int),
+        ("site_records", ctypes.c_int),
+        ("site_depth", ctypes.c_int),
     ]
 
 

@@ -71,7 +71,7 @@ typedef struct vd_config {
   int clip_blocks;    /* workgroups of the site-major launch; 0 = automatic */
   int site_tcap;      /* vertex cap of the site-major launch, at most 32; pairs that
                          exceed it are recomputed at tcap, so results do not depend
-                         on it; 0 = automatic (min(tcap, 32)) */
+                         on it; 0 = automatic (min(tcap, 16)) */
   int cull_depth;     /* site-major form: a pair whose tet lies entirely beyond one of the
                          seed's first cull_depth bisectors (by more than the FP32 rounding
                          bound plus 0.01) has an empty intersection and is not clipped;
@@ -87,6 +87,14 @@ typedef struct vd_config {
                          flags needs_exact_predicates); 2 such a determinant is
                          re-evaluated in FP64 from the same planes, its sign decides and
                          only its FP64 bound (voronoi.cu:325-341) flags the pair */
+  int site_records;   /* site-major form: polytopes whose clipping ended go to a separate
+                         volume pass through a buffer of this many records (16 bytes plus 4
+                         per vertex of the launch cap, rounded up to 4 vertices); past it a
+                         lane computes its volume in place, so results do not depend on it;
+                         0 = automatic (n_pairs / 10 + 262144), -1 = every volume in place */
+  int site_depth;     /* site-major form: bisectors per site table of the launch; a pair that
+                         needs more is recomputed from scratch at site_k by the retry pass,
+                         so results do not depend on it; 0 = automatic */
 } vd_config;
 
 typedef struct vd_timing {

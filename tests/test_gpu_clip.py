@@ -99,6 +99,42 @@ def test_site_form_overflow_retry(vdn, O, site_tcap):
     assert [g["hist"][k] for k in vdn.STATUS_NAMES] == hist.tolist()
 
 
+@pytest.mark.parametrize("records,site_tcap", [(-1, 0), (1, 0), (700, 0), (700, 10)])
+def test_site_form_volume_records(vdn, O, records, site_tcap):
+    """The site-major form hands finished polytopes to a separate volume pass
+    through a record buffer and computes the volume in place past its
+    capacity (-1: always in place; 1 and 700: mostly in place, partly by
+    records; site_tcap 10: records of 12 triangle words).  Every pair's
+    result and the site sums are the oracle's, whichever pass computed them."""
+    verts, tets, sites = problem("cube", 20, 2000)
+    sk, tk = oracle_knn("cube", 20, 2000, 90, 64)
+    print(f"records={records} site_tcap={site_tcap}", flush=True)
+    g = _run(vdn, verts, tets, sites, tet_k=64, form="site", accum="f64", site_records=records, site_tcap=site_tcap)
+    r = O.clip_tets(sites, sk, verts, tets, tk, geom="f32", pair_acc="f64", site_mode=O.SITE_FIXED)
+    np.testing.assert_array_equal(g["status"].astype(np.int32), r["pair_status"])
+    np.testing.assert_array_equal(g["vals"], r["pair_vals"])
+    np.testing.assert_array_equal(g["moments"], r["site_vals"])
+    hist = np.bincount(r["pair_status"], minlength=7)
+    assert [g["hist"][k] for k in vdn.STATUS_NAMES] == hist.tolist()
+
+
+@pytest.mark.parametrize("depth", [6, 16])
+def test_site_form_depth_retry(vdn, O, depth):
+    """Site tables shorter than the neighbour lists: a pair that needs more
+    bisectors is recomputed from scratch at the full depth, so every result
+    equals the oracle's (depth 6: most non-empty pairs take the retry)."""
+    verts, tets, sites = problem("cube", 20, 2000)
+    sk, tk = oracle_knn("cube", 20, 2000, 90, 64)
+    print(f"site_depth={depth}", flush=True)
+    g = _run(vdn, verts, tets, sites, tet_k=64, form="site", accum="f64", site_depth=depth)
+    r = O.clip_tets(sites, sk, verts, tets, tk, geom="f32", pair_acc="f64", site_mode=O.SITE_FIXED)
+    np.testing.assert_array_equal(g["status"].astype(np.int32), r["pair_status"])
+    np.testing.assert_array_equal(g["vals"], r["pair_vals"])
+    np.testing.assert_array_equal(g["moments"], r["site_vals"])
+    hist = np.bincount(r["pair_status"], minlength=7)
+    assert [g["hist"][k] for k in vdn.STATUS_NAMES] == hist.tolist()
+
+
 def test_accuracy_vs_fp64_oracle(vdn, O):
     """FP32 clipping + FP64 accumulation vs the all-FP64 oracle."""
     verts, tets, sites = problem("cube", 20, 2000)
