@@ -720,8 +720,9 @@ __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
     };
     acc = run_pair<MF, ACC>(ak, c, nbr, plane_of, S);
     status = c.status;
-    if (status == kTriangleOverflow && a.retry_list && a.tcap < a.tcap_full) {
-      // over this launch's vertex cap: recomputed by a launch at a wider one
+    if (a.retry_list && ((status == kTriangleOverflow && a.tcap < a.tcap_full) ||
+                         (status == kVertexOverflow && a.pcap < a.pcap_full))) {
+      // over one of this launch's caps: recomputed by a launch at wider ones
       const unsigned slot = atomicAdd(a.retry_count, 1u);
       if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = (unsigned)pair;
       status = -1;

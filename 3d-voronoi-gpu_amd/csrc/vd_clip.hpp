@@ -22,6 +22,8 @@ struct ClipArgs {
   int pcap, tcap;            // polytope caps (planes, vertices) of this launch
   int tcap_full;             // configured vertex cap (site-major form: pairs that
                              // overflow a smaller launch cap go to retry_list)
+  int pcap_full;             // configured plane cap (tet-major form over a retry list:
+                             // likewise for a smaller launch plane cap)
   unsigned* retry_list;      // optional, n_pairs entries
   unsigned* retry_count;     // with retry_list
   int cull_depth;            // site-major form: bisectors tried by the emptiness test (0 = off)

@@ -714,27 +714,31 @@ int vd_clip_step(vd_ctx* ctx) {
   VD_HIP(vd::clip_launch(a, c.form, c.moments, c.accum, blocks, ctx->stream), "clip");
   VD_HIP(record_phase(ctx, (5)), "event");
   if (a.retry_list) {
-    // pairs that overflowed the site-major launch's vertex cap: re-clipped by
-    // the tet-major kernel with cached vertex distances at a cap of 32, and
-    // those that overflow that too by the plain one at the configured caps
+    // pairs that overflowed the site-major launch's vertex cap or needed more
+    // bisectors than its tables hold: re-clipped by the tet-major kernel with
+    // cached vertex distances at caps of 32 planes and 32 vertices (3 waves
+    // per CU instead of 1 at the configured 64 planes), and those that
+    // overflow that too by the plain one at the configured caps
     vd::ClipArgs r = a;
     r.pair_list = a.retry_list;
     r.pair_list_count = a.retry_count;
     r.retry_list = nullptr;
     r.retry_count = nullptr;
-    const int mid = std::min(c.tcap, 32);
-    if (mid > a.tcap && vd::clip_plain_cached_lds(c.pcap, mid) <= 160 * 1024) {
+    const int mid = std::min(c.tcap, 32), pmid = std::min(c.pcap, 32);
+    if (mid >= a.tcap && vd::clip_plain_cached_lds(pmid, mid) <= 160 * 1024) {
       r.tcap = mid;
       r.tcap_full = c.tcap;
-      if (mid < c.tcap) {
+      r.pcap = pmid;
+      r.pcap_full = c.pcap;
+      if (mid < c.tcap || pmid < c.pcap) {
         VD_HIP(ctx->retry_list2.ensure(std::max(n_pairs, 1LL)), "alloc retry list");
         VD_HIP(ctx->retry_count2.ensure(1), "alloc retry list");
         VD_HIP(vd::fill_u32(ctx->retry_count2.p, 1, 0u, ctx->stream), "clear retry count");
         r.retry_list = ctx->retry_list2.p;
         r.retry_count = ctx->retry_count2.p;
       }
-      VD_HIP(vd::clip_launch(r, vd::kFormTetCached, c.moments, c.accum, 256, ctx->stream), "clip retry");
-      if (mid < c.tcap) {
+      VD_HIP(vd::clip_launch(r, vd::kFormTetCached, c.moments, c.accum, 768, ctx->stream), "clip retry");
+      if (mid < c.tcap || pmid < c.pcap) {
         vd::ClipArgs r3 = a;
         r3.tcap = c.tcap;
         r3.pair_list = ctx->retry_list2.p;
