@@ -48,7 +48,7 @@ N_CUBE, N_SITES, TK, K = 55, 100000, 64, 90
 W_PAIR, W_CLIP, W_CONFLICT, W_SECURE_VTX, W_SECURE_CHECK, W_ACC_VTX = 80, 14, 45, 67, 9, 545
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_clip_site_c2_hbm_traffic.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "clip_site_998k_tets_100k_sites_hbm_traffic.md")
 
 
 def log(*a):
@@ -288,14 +288,14 @@ def main():
         fpp = flops_per_pair(cpu["events"])
         log(f"cpu baseline: {cpu['iters_per_s']:.4f} iters/s on {cpu['cores']} threads; {fpp:.0f} FLOP/pair")
 
-    # the site-major clipping kernel alone: the clip phase minus the
-    # re-clipping of pairs that overflowed its vertex cap (HIP events on the
-    # engine's stream, around the launch)
+    # the site-major clipping kernel and its volume pass: the clip phase minus
+    # the re-clipping of pairs that overflowed its caps (HIP events on the
+    # engine's stream, around the two launches)
     kern_s = (ts["clip_ms"] - ts["overflow_ms"]) / n / 1e3
     roof = None
     if fpp is not None:
         achieved = fpp * n_pairs / kern_s / 1e12
-        roof = {"bound": "valu", "kernel": "k_clip_site (clipping + accumulation)", "achieved": round(achieved, 3),
+        roof = {"bound": "valu", "kernel": "k_clip_site + k_site_moments (clipping; volumes, moments, accumulation)", "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
                 "traffic": profiled_traffic(), "flop_per_launch": fpp * n_pairs, "kernel_ms": round(kern_s * 1e3, 3)}
     value = args.steps / wall
