@@ -348,78 +348,53 @@ struct Cell {
     }
   }
 
-  // Clip by plane eqn stored under id `pid` (voronoi.cu:443-486); the plane is
-  // the bisector of the seed with a site at squared distance d2z.
-  //
-  // Returns the outcome of the security-radius test that follows the clip in
-  // kernel A (d2z > 4 max_t |x_t - seed|^2) when the cached form knows it (1
-  // or 0), or -1 when the caller has to evaluate it.
-  __device__ int clip(int pid, const P4& eqn, float d2z) {
+  // Cached form of the clip in two steps, so that a caller can test a lane's
+  // next bisectors one after another and run the expensive cut for many lanes
+  // at once.  clip_test: the plane-cap check, the conflict flags and the
+  // security outcome; returns as clip() when the plane removes no vertex (or
+  // flags needs_exact_predicates, or overflows), else 2 with the flags in
+  // conf_out and the security outcome of the kept vertices in secure_out.
+  __device__ __forceinline__ int clip_test(const P4& eqn, float d2z, uint32_t& conf_out, bool& secure_out) {
     if (nv >= pcap) {
       status = kVertexOverflow;
       return -1;
     }
     ++nv;
     nr = 0;
-    if constexpr (TS::kCached) {
-      // A vertex within 0.9 * |z - s| / 2 of the seed (d2 < 0.2025 d2z) is
-      // closer to s than to z by at least 0.05 |z - s|, far beyond the FP32
-      // error of the determinant, so the test's outcome (no conflict) is known
-      // without computing it.  d2z does not decrease along the neighbour list,
-      // so a vertex found inside that ball stays inside for the later
-      // bisectors: `far` keeps the slots not yet found inside.
-      // Security: a vertex blocks it when 4 d2 >= d2z (false for NaN, which
-      // the reference's fmaxf skips); only far vertices can block.
-      if (!(d2z >= last_d2z)) far = (1u << nt) - 1u;
-      last_d2z = d2z;
-      // (with the filter on every determinant is evaluated, as in the reference)
-      const float near2 = filter ? -1.f : 0.2025f * d2z;
-      uint32_t cand = 0, block = 0;
-      for (uint32_t m = far; m; m &= m - 1) {
-        const int t = __builtin_ctz(m);
-        const float d = ts.d2(t);
-        if (d < near2) {
-          far &= ~(1u << t);
-        } else {
-          cand |= 1u << t;
-          if (4.f * d >= d2z) block |= 1u << t;
-        }
+    // A vertex within 0.9 * |z - s| / 2 of the seed (d2 < 0.2025 d2z) is
+    // closer to s than to z by at least 0.05 |z - s|, far beyond the FP32
+    // error of the determinant, so the test's outcome (no conflict) is known
+    // without computing it.  d2z does not decrease along the neighbour list,
+    // so a vertex found inside that ball stays inside for the later
+    // bisectors: `far` keeps the slots not yet found inside.
+    // Security: a vertex blocks it when 4 d2 >= d2z (false for NaN, which
+    // the reference's fmaxf skips); only far vertices can block.
+    if (!(d2z >= last_d2z)) far = (1u << nt) - 1u;
+    last_d2z = d2z;
+    // (with the filter on every determinant is evaluated, as in the reference)
+    const float near2 = filter ? -1.f : 0.2025f * d2z;
+    uint32_t cand = 0, block = 0;
+    for (uint32_t m = far; m; m &= m - 1) {
+      const int t = __builtin_ctz(m);
+      const float d = ts.d2(t);
+      if (d < near2) {
+        far &= ~(1u << t);
+      } else {
+        cand |= 1u << t;
+        if (4.f * d >= d2z) block |= 1u << t;
       }
-      // the determinant only for candidates: a wave pays for the largest
-      // candidate count rather than the largest vertex count
-      uint32_t conf = 0;
-      for (uint32_t m = cand; m; m &= m - 1) {
-        const int t = __builtin_ctz(m);
-        if (conflict(tri(t), eqn)) conf |= 1u << t;
-      }
-      const bool kept_secure = d2z > 0.f && (block & ~conf) == 0u;
-      if (status == kNeedsExactPredicates) {
-        // the reference stops right after its conflict loop (voronoi.cu:459-461):
-        // the conflicting vertices are gone, no plane is dropped, no fan
-        while (conf) {
-          const int i = __builtin_ctz(conf);
-          --nt;
-          ++nr;
-          const uint32_t bi = 1u << i, bn = 1u << nt;
-          if (i != nt) {
-            ts.swap(i, nt);
-            conf = (conf & ~(bi | bn)) | ((conf & bn) ? bi : 0u);
-            far = (far & ~(bi | bn)) | ((far & bn) ? bi : 0u);
-          } else {
-            conf &= ~bi;
-            far &= ~bi;
-          }
-        }
-        return -1;
-      }
-      if (conf == 0u) {
-        --nv;
-        return kept_secure ? 1 : 0;
-      }
-      // The reference's loop (ascending slot, a conflicting slot takes the
-      // last live triangle and is tested again) replayed from the flags: a
-      // flag depends on the triangle's planes only, so the resulting order is
-      // the one the in-loop tests produce; each trip removes one triangle.
+    }
+    // the determinant only for candidates: a wave pays for the largest
+    // candidate count rather than the largest vertex count
+    uint32_t conf = 0;
+    for (uint32_t m = cand; m; m &= m - 1) {
+      const int t = __builtin_ctz(m);
+      if (conflict(tri(t), eqn)) conf |= 1u << t;
+    }
+    const bool kept_secure = d2z > 0.f && (block & ~conf) == 0u;
+    if (status == kNeedsExactPredicates) {
+      // the reference stops right after its conflict loop (voronoi.cu:459-461):
+      // the conflicting vertices are gone, no plane is dropped, no fan
       while (conf) {
         const int i = __builtin_ctz(conf);
         --nt;
@@ -434,38 +409,91 @@ struct Cell {
           far &= ~bi;
         }
       }
-      if (nt == 0) {
-        status = kNoIntersection;
+      return -1;
+    }
+    if (conf == 0u) {
+      --nv;
+      return kept_secure ? 1 : 0;
+    }
+    conf_out = conf;
+    secure_out = kept_secure;
+    return 2;
+  }
+
+  // clip_cut: removal of the flagged vertices, the cavity walk and the fan of
+  // new vertices around plane `pid`; returns as clip().
+  __device__ __forceinline__ int clip_cut(int pid, uint32_t conf, bool kept_secure, float d2z) {
+    const float near2 = filter ? -1.f : 0.2025f * d2z;
+    // The reference's loop (ascending slot, a conflicting slot takes the
+    // last live triangle and is tested again) replayed from the flags: a
+    // flag depends on the triangle's planes only, so the resulting order is
+    // the one the in-loop tests produce; each trip removes one triangle.
+    while (conf) {
+      const int i = __builtin_ctz(conf);
+      --nt;
+      ++nr;
+      const uint32_t bi = 1u << i, bn = 1u << nt;
+      if (i != nt) {
+        ts.swap(i, nt);
+        conf = (conf & ~(bi | bn)) | ((conf & bn) ? bi : 0u);
+        far = (far & ~(bi | bn)) | ((far & bn) ? bi : 0u);
+      } else {
+        conf &= ~bi;
+        far &= ~bi;
+      }
+    }
+    if (nt == 0) {
+      status = kNoIntersection;
+      return -1;
+    }
+    cavity_boundary();
+    if (status != kSuccess) return -1;
+    if (first == kNoPlane) return kept_secure ? 1 : 0;
+    bool new_block = false;
+    int c = first;
+    int guard = 0;
+    do {
+      if (c == kNoPlane || guard++ > pcap) {
+        status = kInconsistentBoundary;
         return -1;
       }
-      cavity_boundary();
-      if (status != kSuccess) return -1;
-      if (first == kNoPlane) return kept_secure ? 1 : 0;
-      bool new_block = false;
-      int c = first;
-      int guard = 0;
-      do {
-        if (c == kNoPlane || guard++ > pcap) {
-          status = kInconsistentBoundary;
-          return -1;
-        }
-        if (nt + 1 >= tcap) {
-          status = kTriangleOverflow;
-          return -1;
-        }
-        const int n = nxt(c);
-        if (n == kNoPlane) {
-          status = kInconsistentBoundary;
-          return -1;
-        }
-        const float d = put_tri(nt, pack3(pid, c, n));
-        if (!(d < near2)) far |= 1u << nt;
-        new_block = new_block || (4.f * d >= d2z);
-        ++nt;
-        c = n;
-      } while (c != first);
-      return (kept_secure && !new_block) ? 1 : 0;
+      if (nt + 1 >= tcap) {
+        status = kTriangleOverflow;
+        return -1;
+      }
+      const int n = nxt(c);
+      if (n == kNoPlane) {
+        status = kInconsistentBoundary;
+        return -1;
+      }
+      const float d = put_tri(nt, pack3(pid, c, n));
+      if (!(d < near2)) far |= 1u << nt;
+      new_block = new_block || (4.f * d >= d2z);
+      ++nt;
+      c = n;
+    } while (c != first);
+    return (kept_secure && !new_block) ? 1 : 0;
+  }
+
+  // Clip by plane eqn stored under id `pid` (voronoi.cu:443-486); the plane is
+  // the bisector of the seed with a site at squared distance d2z.
+  //
+  // Returns the outcome of the security-radius test that follows the clip in
+  // kernel A (d2z > 4 max_t |x_t - seed|^2) when the cached form knows it (1
+  // or 0), or -1 when the caller has to evaluate it.
+  __device__ int clip(int pid, const P4& eqn, float d2z) {
+    if constexpr (TS::kCached) {
+      uint32_t conf = 0;
+      bool kept_secure = false;
+      const int r = clip_test(eqn, d2z, conf, kept_secure);
+      return r != 2 ? r : clip_cut(pid, conf, kept_secure, d2z);
     } else {
+      if (nv >= pcap) {
+        status = kVertexOverflow;
+        return -1;
+      }
+      ++nv;
+      nr = 0;
       int i = 0;
       while (i < nt) {
         if (conflict(tri(i), eqn)) {
@@ -770,7 +798,16 @@ __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
 constexpr int kQueue = 128;           // ring of queued (pair, slot) entries
 constexpr float kInvalidNbr = -1.f;   // table mark of an out-of-range neighbour id
 constexpr int kIdle = 0, kClipping = 1;  // lane states
+// bisectors a lane may test per trip before the trip's cuts run
+#ifndef VD_TESTS_PER_TRIP
+#define VD_TESTS_PER_TRIP 3
+#endif
+constexpr int kTestsPerTrip = VD_TESTS_PER_TRIP;
 constexpr int kRecChunk = 128;           // record slots a wave reserves at a time
+// Records of polytopes with more than kRecSplit vertices go to chunks of their
+// own, so that a wave of the volume pass (64 records of one chunk) runs over
+// polytopes of similar size.
+constexpr int kRecSplit = 9;
 
 // Corners of a tet from its record (x0 y0 z0 x1 | y1 z1 x2 y2 | z2 x3 y3 z3).
 __device__ __forceinline__ void corners_of(const float4& q0, const float4& q1, const float4& q2, P4 v[4]) {
@@ -849,8 +886,8 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   unsigned pf_pair = 0, pn_pair = 0;
   float4 pf_r0 = {}, pf_r1 = {}, pf_r2 = {};
 
-  // record slots reserved by this wave: [rec_next, rec_end)
-  unsigned rec_next = 0, rec_end = 0;  // wave-uniform
+  // record slots reserved by this wave, per size class: [rec_next, rec_end)
+  unsigned rec_next0 = 0, rec_end0 = 0, rec_next1 = 0, rec_end1 = 0;  // wave-uniform
 
   for (;;) {
     // (1) refill the queue up to 64 entries
@@ -1074,45 +1111,58 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
 
     bool defer = false;
     if (lst == kClipping) {
-      // (3) one bisector for every clipping lane: kernel A's loop
-      // (voronoi.cu:679-700) one trip at a time
-      bool done = false, fin = false, redo = false;
+      // (3) kernel A's loop (voronoi.cu:679-700), a few bisectors per trip: a
+      // lane tests its next bisectors until one removes a vertex of its
+      // polytope (at most kTestsPerTrip of them), then the cuts of all lanes
+      // run together, so that the costly cut is not run for one lane while
+      // the others wait
+      bool cut = false;
+      uint32_t conf = 0;
+      bool ksec = false;
+      float cut_d2z = 0.f;
+      // The loop's exits after the clip of bisector pv: 0 go on, 1 finished
+      // (the volume pass follows), 2 ended without one, 3 recomputed deeper.
+      auto step_end = [&](int sec, float d2z) -> int {
+        if (sec >= 0 ? sec != 0 : c.security_reached(d2z)) return 1;
+        if (c.status != kSuccess) return 2;
+        if (++pv == K) {
+          if (deeper) return 3;
+          c.status = kSecurityRadiusNotReached;
+          return 1;
+        }
+        return 0;
+      };
+      int out = 0;
       if (K == 0) {
         c.status = kSecurityRadiusNotReached;
-        done = fin = true;
+        out = 1;
       } else {
-        const float d2z = bd2[pslot * K4 + pv];
-        if (d2z == kInvalidNbr) {
-          c.status = kInconsistentBoundary;
-          done = true;
-        } else {
-          const int sec = c.clip(4 + pv, ld4(c.ps.bis, pv), d2z);
-          if (sec >= 0 ? sec != 0 : c.security_reached(d2z)) {
-            done = fin = true;
-          } else if (c.status != kSuccess) {
-            done = true;
-          } else if (++pv == K) {
-            if (deeper) {
-              redo = true;
-            } else {
-              c.status = kSecurityRadiusNotReached;
-              fin = true;
-            }
-            done = true;
+        for (int it = 0; it < kTestsPerTrip && out == 0; ++it) {
+          const float d2z = bd2[pslot * K4 + pv];
+          if (d2z == kInvalidNbr) {
+            c.status = kInconsistentBoundary;
+            out = 2;
+            break;
           }
+          const int r = c.clip_test(ld4(c.ps.bis, pv), d2z, conf, ksec);
+          if (r == 2) {
+            cut = true;
+            cut_d2z = d2z;
+            break;
+          }
+          out = step_end(r, d2z);
         }
       }
-      if (done) {
-        if (redo) {
-          const unsigned slot = atomicAdd(a.retry_count, 1u);
-          if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = ppair;
-          lst = kIdle;
-        } else if (fin && c.status != kNoIntersection) {
-          defer = true;
-        } else {
-          PairSums<ACC> S;
-          finish(S, false);
-        }
+      if (cut) out = step_end(c.clip_cut(4 + pv, conf, ksec, cut_d2z), cut_d2z);
+      if (out == 3) {
+        const unsigned slot = atomicAdd(a.retry_count, 1u);
+        if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = ppair;
+        lst = kIdle;
+      } else if (out == 1 && c.status != kNoIntersection) {
+        defer = true;
+      } else if (out != 0) {
+        PairSums<ACC> S;
+        finish(S, false);
       }
     }
     // (4) records of the polytopes for the volume pass, one slot per
@@ -1122,35 +1172,46 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
     if (dm != 0) {
       unsigned slot = ~0u;
       if (a.rec_meta) {
-        const unsigned need = (unsigned)__popcll(dm);
-        const unsigned rank =
-            (unsigned)__builtin_amdgcn_mbcnt_hi((uint32_t)(dm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)dm, 0));
-        const unsigned have = rec_end - rec_next;
-        if (have >= need) {
-          slot = rec_next + rank;
-          rec_next += need;
-        } else {
-          const int leader = __builtin_ctzll(dm);
-          unsigned base = 0;
-          if (lane == leader) base = atomicAdd(a.rec_count, (unsigned)kRecChunk);
-          base = __shfl(base, leader);
-          slot = rank < have ? rec_next + rank : base + (rank - have);
-          rec_next = base + (need - have);
-          rec_end = base + (unsigned)kRecChunk;
-        }
+        const unsigned long long dm1 = __ballot(defer && c.nt > kRecSplit);
+        const unsigned long long dm0 = dm & ~dm1;
+        // slot of this lane among the lanes of mask m (m wave-uniform)
+        auto reserve = [&](unsigned long long m, unsigned& next, unsigned& end) {
+          if (m == 0) return;
+          const unsigned need = (unsigned)__popcll(m);
+          const unsigned rk =
+              (unsigned)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+          const unsigned have = end - next;
+          const bool mine = (m >> lane) & 1ull;
+          if (have >= need) {
+            if (mine) slot = next + rk;
+            next += need;
+          } else {
+            const int leader = __builtin_ctzll(m);
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(a.rec_count, (unsigned)kRecChunk);
+            base = __shfl(base, leader);
+            if (mine) slot = rk < have ? next + rk : base + (rk - have);
+            next = base + (need - have);
+            end = base + (unsigned)kRecChunk;
+          }
+        };
+        reserve(dm0, rec_next0, rec_end0);
+        reserve(dm1, rec_next1, rec_end1);
       }
       if (defer) {
         if (a.rec_meta && slot < a.rec_cap) {
           a.rec_meta[slot] = make_uint4(ppair, (unsigned)(pslot ? site1 : site0), ptet,
                                         (unsigned)c.nt | ((unsigned)c.status << 8));
           uint4* dst = (uint4*)(a.rec_tris + (size_t)slot * rec_stride(a.tcap));
-          for (int t = 0; t < c.nt; t += 4) {
-            uint4 w;
-            w.x = c.tri(t);
-            w.y = t + 1 < c.nt ? c.tri(t + 1) : 0u;
-            w.z = t + 2 < c.nt ? c.tri(t + 2) : 0u;
-            w.w = t + 3 < c.nt ? c.tri(t + 3) : 0u;
-            dst[t >> 2] = w;
+          // all LDS reads of a group of 16 first, then the stores (words past
+          // nt are not read by the volume pass)
+          for (int t0 = 0; t0 < c.nt; t0 += 16) {
+            uint32_t w[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) w[j] = c.tri(min(t0 + j, c.tcap - 1));
+#pragma unroll
+            for (int j = 0; j < 16; j += 4)
+              if (t0 + j < c.nt) dst[(t0 + j) >> 2] = make_uint4(w[j], w[j + 1], w[j + 2], w[j + 3]);
           }
           lst = kIdle;
         } else {
@@ -1160,8 +1221,10 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
     }
   }
   // reserved record slots left unused are marked for the volume pass
-  if (a.rec_meta)
-    for (unsigned r = rec_next + lane; r < rec_end && r < a.rec_cap; r += 64) a.rec_meta[r] = make_uint4(~0u, 0u, 0u, 0u);
+  if (a.rec_meta) {
+    for (unsigned r = rec_next0 + lane; r < rec_end0 && r < a.rec_cap; r += 64) a.rec_meta[r] = make_uint4(~0u, 0u, 0u, 0u);
+    for (unsigned r = rec_next1 + lane; r < rec_end1 && r < a.rec_cap; r += 64) a.rec_meta[r] = make_uint4(~0u, 0u, 0u, 0u);
+  }
   __syncthreads();
   if (a.status_hist && lane < 7) {
     const unsigned n = hist[lane] + (lane == kNoIntersection ? n_culled : 0u);

@@ -296,7 +296,7 @@ void vd_config_default(vd_config* c) {
   c->record_pairs = 0;
   c->clip_blocks = 0;
   c->site_tcap = 0;
-  c->cull_depth = 32;
+  c->cull_depth = 16;
   c->srnr_k = 0;
   c->filter = 0;
   c->site_records = 0;

@@ -112,8 +112,8 @@ typedef struct vd_ctx vd_ctx;
 
 /* Defaults: site_k 90, tet_k 0, pcap 64, tcap 96, VD_ACCUM_F32,
  * VD_MOMENTS_REFERENCE, VD_FORM_SITE_MAJOR, grid_density 3.1
- * (knearests.cu:243), record_pairs 0, clip_blocks 0, site_tcap 0, cull_depth 32, srnr_k 0,
- * filter 0. */
+ * (knearests.cu:243), record_pairs 0, clip_blocks 0, site_tcap 0, cull_depth 16, srnr_k 0,
+ * filter 0, site_records 0, site_depth 0. */
 void vd_config_default(vd_config* cfg);
 
 /* Context on HIP device `device`.  Replaces the device selection and buffer

@@ -27,7 +27,8 @@ def test_library_exports_every_declared_function(vdn):
 def test_config_layout_and_defaults(vdn):
     c = vdn.default_config()
     assert (c.site_k, c.tet_k, c.pcap, c.tcap) == (90, 0, 64, 96)
-    assert abs(c.grid_density - 3.1) < 1e-6 and c.cull_depth == 32
+    assert abs(c.grid_density - 3.1) < 1e-6 and c.cull_depth == 16
+    assert (c.site_tcap, c.srnr_k, c.filter, c.site_records, c.site_depth) == (0, 0, 0, 0, 0)
     src = open(os.path.join(ROOT, "include", "vd_api.h")).read()
     body = src[src.index("typedef struct vd_config {"):src.index("} vd_config;")]
     fields = re.findall(r"^\s*(?:int|float)\s+(\w+);", body, flags=re.M)
