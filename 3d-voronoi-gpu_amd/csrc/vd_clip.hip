@@ -36,6 +36,15 @@ __device__ __forceinline__ int tri_b(uint32_t t) { return (int)((t >> 8) & 255u)
 __device__ __forceinline__ int tri_c(uint32_t t) { return (int)((t >> 16) & 255u); }
 __device__ __forceinline__ int tri_at(uint32_t t, int k) { return (int)((t >> (8 * k)) & 255u); }
 
+// Slots per trip of the cached clip's distance loop and determinant loop.
+#ifndef VD_FAR_UNROLL
+#define VD_FAR_UNROLL 4
+#endif
+#ifndef VD_CAND_UNROLL
+#define VD_CAND_UNROLL 4
+#endif
+constexpr int kFarUnroll = VD_FAR_UNROLL, kCandUnroll = VD_CAND_UNROLL;
+
 // Storage policies --------------------------------------------------------
 //
 // PlaneStoreLocal: planes [pcap][64] per lane, ids are positions (reference).
@@ -374,22 +383,50 @@ struct Cell {
     // (with the filter on every determinant is evaluated, as in the reference)
     const float near2 = filter ? -1.f : 0.2025f * d2z;
     uint32_t cand = 0, block = 0;
-    for (uint32_t m = far; m; m &= m - 1) {
-      const int t = __builtin_ctz(m);
-      const float d = ts.d2(t);
-      if (d < near2) {
-        far &= ~(1u << t);
-      } else {
-        cand |= 1u << t;
-        if (4.f * d >= d2z) block |= 1u << t;
+    // kFarUnroll slots per loop trip, so that their LDS reads are in flight
+    // together
+    for (uint32_t m = far; m;) {
+      int t[kFarUnroll];
+      bool has[kFarUnroll];
+      float d[kFarUnroll];
+#pragma unroll
+      for (int j = 0; j < kFarUnroll; ++j) {
+        has[j] = m != 0;
+        t[j] = has[j] ? __builtin_ctz(m) : 0;
+        m &= m - 1;
+      }
+#pragma unroll
+      for (int j = 0; j < kFarUnroll; ++j) d[j] = ts.d2(t[j]);
+#pragma unroll
+      for (int j = 0; j < kFarUnroll; ++j) {
+        if (!has[j]) continue;
+        if (d[j] < near2) {
+          far &= ~(1u << t[j]);
+        } else {
+          cand |= 1u << t[j];
+          if (4.f * d[j] >= d2z) block |= 1u << t[j];
+        }
       }
     }
     // the determinant only for candidates: a wave pays for the largest
-    // candidate count rather than the largest vertex count
+    // candidate count rather than the largest vertex count (kCandUnroll per
+    // trip)
     uint32_t conf = 0;
-    for (uint32_t m = cand; m; m &= m - 1) {
-      const int t = __builtin_ctz(m);
-      if (conflict(tri(t), eqn)) conf |= 1u << t;
+    for (uint32_t m = cand; m;) {
+      int t[kCandUnroll];
+      bool has[kCandUnroll];
+      uint32_t tt[kCandUnroll];
+#pragma unroll
+      for (int j = 0; j < kCandUnroll; ++j) {
+        has[j] = m != 0;
+        t[j] = has[j] ? __builtin_ctz(m) : 0;
+        m &= m - 1;
+      }
+#pragma unroll
+      for (int j = 0; j < kCandUnroll; ++j) tt[j] = tri(t[j]);
+#pragma unroll
+      for (int j = 0; j < kCandUnroll; ++j)
+        if (has[j] && conflict(tt[j], eqn)) conf |= 1u << t[j];
     }
     const bool kept_secure = d2z > 0.f && (block & ~conf) == 0u;
     if (status == kNeedsExactPredicates) {
