@@ -489,6 +489,10 @@ struct Cell {
     bool new_block = false;
     int c = first;
     int guard = 0;
+    // the new vertex (pid, c, n): the plane of pid and of c are already in
+    // registers (c is the previous vertex's n), one plane read per vertex
+    const P4 pp = ps.get(pid);
+    P4 pc = ps.get(first);
     do {
       if (c == kNoPlane || guard++ > pcap) {
         status = kInconsistentBoundary;
@@ -503,11 +507,14 @@ struct Cell {
         status = kInconsistentBoundary;
         return -1;
       }
-      const float d = put_tri(nt, pack3(pid, c, n));
+      const P4 pn = ps.get(n);
+      const float d = seed_d2(vertex3(pp, pc, pn));
+      ts.put(nt, pack3(pid, c, n), d);
       if (!(d < near2)) far |= 1u << nt;
       new_block = new_block || (4.f * d >= d2z);
       ++nt;
       c = n;
+      pc = pn;
     } while (c != first);
     return (kept_secure && !new_block) ? 1 : 0;
   }
