@@ -821,13 +821,13 @@ __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
 
 // ----------------------------------------------------------- site-major ----
 //
-// Pairs grouped by seed: site s owns pair_ids[pair_start[s] .. pair_start[s+1]).
-// A wavefront takes sites one at a time from a global counter.  For each site
-// it builds the bisector table once (shared by the wave), runs the emptiness
-// test over the site's pairs 64 at a time and queues the survivors in an LDS
-// ring.  Every lane clips its own pair one bisector per loop trip and takes
-// the next queued pair as soon as it is done, so lanes do not wait for the
-// slowest pair of a group.  Two table slots let lanes start on the next site
+// Pairs grouped by seed: site s owns pair_ids[pair_start[s] .. pair_start[s+1]),
+// or the first pair_count[s] of them when the emptiness test (k_cull_site)
+// has run.  A wavefront takes sites one at a time from a global counter.  For
+// each site it builds the bisector table once (shared by the wave) and queues
+// the site's pairs in an LDS ring.  Every lane clips its own pair a few
+// bisectors per loop trip and takes the next queued pair as soon as it is
+// done, so lanes do not wait for the slowest pair of a group.  Two table slots let lanes start on the next site
 // while others finish the previous one.  Tet corners come from a per-tet
 // record (3 float4 per tet), one dependent load per pair.  A pair sees the
 // same planes in the same order as in kernel A, so its result is the same bit
@@ -888,7 +888,6 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   const int K = a.site_depth > 0 ? min(a.site_depth, a.K) : a.K;
   const bool deeper = K < a.K;
   const int K4 = (K + 3) & ~3;
-  const int depth = min(a.cull_depth, K);
   size_t off[8];
   site_lds_layout(K, a.tcap, off);
   float4* bis = (float4*)(smem + off[0]);
@@ -915,20 +914,11 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   unsigned qhead = 0, qtail = 0;      // queue ring positions
   unsigned qsplit = 0;                // entries before qsplit belong to the other slot
   bool no_more_sites = false;
-  unsigned n_culled = 0;
 
   // lane state
   int lst = kIdle;
   int pv = 0, pslot = 0;
   unsigned ppair = 0, ptet = 0;
-
-  // Loads ahead of the emptiness test: the pair ids and tet records of the
-  // chunk at pf_cursor and the pair ids of the chunk at pn_cursor.  Testing a
-  // chunk issues the next chunk's records and the ids of the one after, so
-  // the loads complete under the tests instead of stalling each chunk.
-  unsigned pf_cursor = ~0u, pn_cursor = ~0u;  // wave-uniform
-  unsigned pf_pair = 0, pn_pair = 0;
-  float4 pf_r0 = {}, pf_r1 = {}, pf_r2 = {};
 
   // record slots reserved by this wave, per size class: [rec_next, rec_end)
   unsigned rec_next0 = 0, rec_end0 = 0, rec_next1 = 0, rec_end1 = 0;  // wave-uniform
@@ -940,106 +930,10 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         __syncthreads();  // queue entries read by the previous trip
         const unsigned i = cursor + lane;
         const bool live = i < feed_end;
-        unsigned pair = 0;
-        float4 r0 = {}, r1 = {}, r2 = {};
-        if (pf_cursor == cursor) {
-          pair = pf_pair;
-          r0 = pf_r0;
-          r1 = pf_r1;
-          r2 = pf_r2;
-        } else if (live) {
-          pair = a.pair_ids[i];
-          if (depth > 0) {
-            const float4* r = a.tet_xyz + 3 * (size_t)(pair / (unsigned)a.TK);
-            r0 = r[0];
-            r1 = r[1];
-            r2 = r[2];
-          }
-        }
-        // the next chunk's records (its ids came with the previous chunk, or
-        // now) and the ids of the chunk after it
-        const unsigned c1 = cursor + 64, c2 = cursor + 128;
-        pf_cursor = ~0u;
-        if (c1 < feed_end) {
-          const bool live1 = c1 + lane < feed_end;
-          pf_pair = pn_cursor == c1 ? pn_pair : (live1 ? a.pair_ids[c1 + lane] : 0u);
-          if (depth > 0 && live1) {
-            const float4* r = a.tet_xyz + 3 * (size_t)(pf_pair / (unsigned)a.TK);
-            pf_r0 = r[0];
-            pf_r1 = r[1];
-            pf_r2 = r[2];
-          }
-          pf_cursor = c1;
-        }
-        pn_cursor = ~0u;
-        if (c2 < feed_end) {
-          pn_pair = c2 + lane < feed_end ? a.pair_ids[c2 + lane] : 0u;
-          pn_cursor = c2;
-        }
-        bool culled = false;
-        if (depth > 0) {
-          P4 v[4] = {};
-          float pmax = 0.f, vmin2 = 0.f;
-          if (live) {
-            corners_of(r0, r1, r2, v);
-            const P4 fs = site_at(a, feed_slot ? site1 : site0);
-            vmin2 = INFINITY;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              pmax = fmaxf(pmax, fmaxf(fabsf(v[j].x), fmaxf(fabsf(v[j].y), fabsf(v[j].z))));
-              const P4 d = sub3(v[j], fs);
-              vmin2 = fminf(vmin2, dot3(d, d));
-            }
-          }
-          const float4* eb = bis + feed_slot * K4;
-          const float* db = bd2 + feed_slot * K4;
-          bool open = live;
-          for (int q0 = 0; q0 < depth; q0 += 4) {
-            if (__ballot(open) == 0) break;
-            if (open) {
-#pragma unroll
-              for (int j = 0; j < 4; ++j) {
-                const int q = q0 + j;
-                if (!open || q >= depth) break;
-                const float4 e = eb[q];
-                const float dd = db[q];
-                if (dd == kInvalidNbr) {  // invalid neighbour: no test, as the clipping would stop there
-                  open = false;
-                } else if (4.f * vmin2 <= dd) {
-                  // the tet's nearest corner is within |z - s| / 2 of the seed,
-                  // so it is on the seed's side of this bisector and of every
-                  // later one (the neighbour distances do not decrease): no
-                  // later test can succeed
-                  open = false;
-                } else {
-                  // rounding bound of the plane evaluation (relative
-                  // kCullRound of |n|_1 * max|x| + |w|) plus the required depth
-                  // beyond the plane (kCullMargin in distance, times |n|)
-                  const float lim = -fmaf(kCullRound, fmaf(fabsf(e.x) + fabsf(e.y) + fabsf(e.z), pmax, fabsf(e.w)),
-                                          kCullMargin * __builtin_amdgcn_sqrtf(dd));
-                  bool all = true;
-#pragma unroll
-                  for (int k = 0; k < 4; ++k)
-                    all = all && (fmaf(e.z, v[k].z, fmaf(e.y, v[k].y, e.x * v[k].x)) + e.w < lim);
-                  if (all) {
-                    culled = true;
-                    open = false;
-                  }
-                }
-              }
-            }
-          }
-          if (culled && a.pair_status) a.pair_status[pair] = (uint8_t)kNoIntersection;
-          if (culled && a.pair_vals) {
-            double* pvals = a.pair_vals + 4 * (size_t)pair;
-            pvals[0] = pvals[1] = pvals[2] = pvals[3] = 0.0;
-          }
-          n_culled += (unsigned)__popcll(__ballot(culled));
-        }
-        const bool keep = live && !culled;
-        const unsigned long long mask = __ballot(keep);
+        const unsigned pair = live ? a.pair_ids[i] : 0u;
+        const unsigned long long mask = __ballot(live);
         const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
-        if (keep) queue[(qtail + (unsigned)below) % kQueue] = make_uint2(pair, (unsigned)feed_slot);
+        if (live) queue[(qtail + (unsigned)below) % kQueue] = make_uint2(pair, (unsigned)feed_slot);
         qtail += (unsigned)__popcll(mask);
         cursor += 64;
         continue;
@@ -1055,7 +949,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         no_more_sites = true;
         break;
       }
-      const unsigned pb = a.pair_start[s], pe = a.pair_start[s + 1];
+      const unsigned pb = a.pair_start[s], pe = a.pair_count ? pb + a.pair_count[s] : a.pair_start[s + 1];
       if (pb == pe) continue;
       const P4 sp = site_at(a, s);
       const int32_t* nbr = a.site_knn + (size_t)s * a.K;
@@ -1270,9 +1164,108 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
     for (unsigned r = rec_next1 + lane; r < rec_end1 && r < a.rec_cap; r += 64) a.rec_meta[r] = make_uint4(~0u, 0u, 0u, 0u);
   }
   __syncthreads();
-  if (a.status_hist && lane < 7) {
-    const unsigned n = hist[lane] + (lane == kNoIntersection ? n_culled : 0u);
-    if (n) atomicAdd(a.status_hist + lane, (unsigned long long)n);
+  if (a.status_hist && lane < 7 && hist[lane]) atomicAdd(a.status_hist + lane, (unsigned long long)hist[lane]);
+}
+
+// Emptiness test of the site-major form as a pass of its own, before the
+// clipping: one wavefront per site builds the site's first `depth` bisectors
+// as the clipping's tables hold them, tests the site's pairs 64 at a time
+// (all four tet corners beyond one of them by more than the FP32 rounding
+// bound plus kCullMargin, with the early exit once the tet's nearest corner
+// is within |z - s| / 2 of the seed) and compacts the survivors, in their
+// order, to the front of the site's pair range; pair_count[s] gets their
+// number.  Small and without polytope state, it runs at full occupancy, where
+// inside the clipping kernel the same test ran at under two waves per SIMD.
+constexpr int kCullMaxDepth = 256;
+
+__global__ __launch_bounds__(64) void k_cull_site(ClipArgs a, int depth, unsigned* pair_ids, unsigned* pair_count) {
+  __shared__ float4 eb[kCullMaxDepth];
+  __shared__ float db[kCullMaxDepth];
+  const int lane = threadIdx.x;
+  for (int s = blockIdx.x; s < a.n_sites; s += gridDim.x) {
+    const unsigned pb = a.pair_start[s], pe = a.pair_start[s + 1];
+    if (pb == pe) {
+      if (lane == 0) pair_count[s] = 0;
+      continue;
+    }
+    const P4 sp = site_at(a, s);
+    const int32_t* nbr = a.site_knn + (size_t)s * a.K;
+    __syncthreads();  // the previous site's readers are done
+    for (int v = lane; v < depth; v += 64) {
+      const int z = nbr[v];
+      if ((unsigned)z >= (unsigned)a.n_sites) {
+        eb[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+        db[v] = kInvalidNbr;
+        continue;
+      }
+      const P4 zp = site_at(a, z);
+      const P4 eq = bisector(sp, zp);
+      const P4 d = sub3(zp, sp);
+      eb[v] = make_float4(eq.x, eq.y, eq.z, eq.w);
+      db[v] = dot3(d, d);
+    }
+    __syncthreads();
+    unsigned w = pb, n_culled = 0;
+    for (unsigned c0 = pb; c0 < pe; c0 += 64) {
+      const unsigned i = c0 + lane;
+      const bool live = i < pe;
+      const unsigned pair = live ? pair_ids[i] : 0u;
+      P4 v[4] = {};
+      float pmax = 0.f, vmin2 = 0.f;
+      if (live) {
+        tet_corners(a, (int)(pair / (unsigned)a.TK), v);
+        vmin2 = INFINITY;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          pmax = fmaxf(pmax, fmaxf(fabsf(v[j].x), fmaxf(fabsf(v[j].y), fabsf(v[j].z))));
+          const P4 d = sub3(v[j], sp);
+          vmin2 = fminf(vmin2, dot3(d, d));
+        }
+      }
+      bool culled = false, open = live;
+      for (int q = 0; q < depth && open; ++q) {
+        const float dd = db[q];
+        if (dd == kInvalidNbr || 4.f * vmin2 <= dd) {
+          // an invalid neighbour ends the clipping there; and once the tet's
+          // nearest corner is within |z - s| / 2 of the seed it is on the
+          // seed's side of this bisector and of every later one (the
+          // neighbour distances do not decrease)
+          open = false;
+        } else {
+          const float4 e = eb[q];
+          // rounding bound of the plane evaluation (relative kCullRound of
+          // |n|_1 * max|x| + |w|) plus the required depth beyond the plane
+          // (kCullMargin in distance, times |n|)
+          const float lim = -fmaf(kCullRound, fmaf(fabsf(e.x) + fabsf(e.y) + fabsf(e.z), pmax, fabsf(e.w)),
+                                  kCullMargin * __builtin_amdgcn_sqrtf(dd));
+          bool all = true;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) all = all && (fmaf(e.z, v[k].z, fmaf(e.y, v[k].y, e.x * v[k].x)) + e.w < lim);
+          if (all) {
+            culled = true;
+            open = false;
+          }
+        }
+      }
+      if (culled && a.pair_status) a.pair_status[pair] = (uint8_t)kNoIntersection;
+      if (culled && a.pair_vals) {
+        double* pvals = a.pair_vals + 4 * (size_t)pair;
+        pvals[0] = pvals[1] = pvals[2] = pvals[3] = 0.0;
+      }
+      n_culled += (unsigned)__popcll(__ballot(culled));
+      // survivors to the front: every position written here has been read
+      // (w never passes the chunk being read)
+      const bool keep = live && !culled;
+      const unsigned long long mask = __ballot(keep);
+      const unsigned below =
+          (unsigned)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+      if (keep) pair_ids[w + below] = pair;
+      w += (unsigned)__popcll(mask);
+    }
+    if (lane == 0) {
+      pair_count[s] = w - pb;
+      if (a.status_hist && n_culled) atomicAdd(a.status_hist + kNoIntersection, (unsigned long long)n_culled);
+    }
   }
 }
 
@@ -1803,6 +1796,16 @@ hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, i
     hipLaunchKernelGGL((k_site_moments<1, 0>), grid, block, 0, st, a);
   else
     hipLaunchKernelGGL((k_site_moments<1, 1>), grid, block, 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t cull_sites(const ClipArgs& a, unsigned* pair_ids, unsigned* pair_count, hipStream_t st) {
+  if (a.n_sites <= 0) return hipSuccess;
+  const int K = a.site_depth > 0 ? std::min(a.site_depth, a.K) : a.K;
+  const int depth = std::min(a.cull_depth, K);
+  if (depth <= 0 || depth > kCullMaxDepth) return hipErrorInvalidValue;
+  const unsigned blocks = (unsigned)std::min(a.n_sites, 65536 * 4);
+  hipLaunchKernelGGL(k_cull_site, dim3(blocks), dim3(64), 0, st, a, depth, pair_ids, pair_count);
   return hipGetLastError();
 }
 

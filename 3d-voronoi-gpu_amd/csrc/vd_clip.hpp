@@ -19,6 +19,8 @@ struct ClipArgs {
   int n_sites;
   const unsigned* pair_start;  // site-major form: n_sites + 1
   const unsigned* pair_ids;  // site-major form: n_pairs pair indices grouped by site
+  const unsigned* pair_count;  // site-major form, optional: only the first pair_count[s]
+                               // pairs of site s's range are clipped (cull_sites' survivors)
   int pcap, tcap;            // polytope caps (planes, vertices) of this launch
   int tcap_full;             // configured vertex cap (site-major form: pairs that
                              // overflow a smaller launch cap go to retry_list)
@@ -26,7 +28,8 @@ struct ClipArgs {
                              // likewise for a smaller launch plane cap)
   unsigned* retry_list;      // optional, n_pairs entries
   unsigned* retry_count;     // with retry_list
-  int cull_depth;            // site-major form: bisectors tried by the emptiness test (0 = off)
+  int cull_depth;            // cull_sites and the cell form: bisectors tried by the emptiness
+                             // test (0 = off)
   int filter;                // arithmetic filter of the conflict test (0 off, 1 reference, 2 FP64 re-check)
   unsigned* site_counter;    // site-major form: next site to take, zero before the launch
   int site_depth;            // site-major form: bisectors per site table (0 = K); a pair
@@ -92,6 +95,13 @@ hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, i
 // n_sites ints, start n_sites + 1, scan_tmp scan_tmp_ints(n_sites).
 hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count, unsigned* start,
                          int* cursor, unsigned* pair_ids, unsigned* scan_tmp, hipStream_t st);
+
+// The site-major form's emptiness test as a pass of its own over the pairs
+// grouped by site (a.pair_start, pair_ids), with a.cull_depth bisectors (at
+// most 256): the survivors of site s are moved, in order, to the front of its
+// range and their number written to pair_count[s]; a culled pair counts as
+// no_intersection (a.status_hist, a.pair_status, a.pair_vals).
+hipError_t cull_sites(const ClipArgs& a, unsigned* pair_ids, unsigned* pair_count, hipStream_t st);
 
 // Rows of the deeper neighbour lists: every site seeding a listed pair gets a
 // row (row_site[r] = s, row_of[s] = r).  row_of must be all -1 and *n_rows 0.

@@ -97,6 +97,7 @@ struct vd_ctx {
   // site-major pairs
   DevBuf<int> p_count, p_cursor;
   DevBuf<unsigned> p_start, p_ids;
+  DevBuf<unsigned> p_surv;  // survivors of the emptiness test per site
   // accumulators
   DevBuf<float> acc_f32;
   DevBuf<unsigned long long> acc_fix;
@@ -352,7 +353,7 @@ void vd_ctx_destroy(vd_ctx* ctx) {
   ctx->g_cell_of.release(); ctx->g_sorted_ids.release(); ctx->g_sorted_pts.release();
   ctx->g_scan_tmp.release(); ctx->p_scan_tmp.release();
   ctx->site_knn.release(); ctx->tet_knn.release();
-  ctx->p_count.release(); ctx->p_start.release(); ctx->p_cursor.release(); ctx->p_ids.release();
+  ctx->p_count.release(); ctx->p_start.release(); ctx->p_cursor.release(); ctx->p_ids.release(); ctx->p_surv.release();
   ctx->acc_f32.release(); ctx->acc_fix.release(); ctx->moments.release();
   ctx->pair_status.release(); ctx->pair_vals.release(); ctx->hist.release();
   ctx->retry_list.release(); ctx->retry_count.release(); ctx->retry_list2.release(); ctx->retry_count2.release(); ctx->site_counter.release();
@@ -679,6 +680,13 @@ int vd_clip_step(vd_ctx* ctx) {
     a.tcap = fast_tcap;
     a.tcap_full = c.tcap;
     a.cull_depth = c.cull_depth;
+    if (std::min(c.cull_depth, site_depth_of(c, ctx->K)) > 0) {
+      // the emptiness test as a pass of its own; the clipping takes the survivors
+      VD_HIP(ctx->p_surv.ensure(n), "alloc pairs");
+      VD_HIP(vd::cull_sites(a, ctx->p_ids.p, ctx->p_surv.p, ctx->stream), "cull");
+      a.pair_count = ctx->p_surv.p;
+      a.cull_depth = 0;
+    }
     if (c.site_records >= 0) {
       // automatic: one pair in ten, plus the slots the waves reserve ahead
       const long long want = c.site_records > 0 ? c.site_records : n_pairs / 10 + 262144;
