@@ -1179,9 +1179,11 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
 constexpr int kCullMaxDepth = 256;
 
 __global__ __launch_bounds__(64) void k_cull_site(ClipArgs a, int depth, unsigned* pair_ids, unsigned* pair_count) {
-  __shared__ float4 eb[kCullMaxDepth];
-  __shared__ float db[kCullMaxDepth];
+  extern __shared__ __align__(16) uint8_t smem[];
+  float4* eb = (float4*)smem;   // [depth] bisector (n, w)
+  float4* ec = eb + depth;      // [depth] |z - s|^2, |n|_1, |w|, kCullMargin * |n|
   const int lane = threadIdx.x;
+  unsigned n_culled = 0;  // over the block's sites: one histogram atomic per block
   for (int s = blockIdx.x; s < a.n_sites; s += gridDim.x) {
     const unsigned pb = a.pair_start[s], pe = a.pair_start[s + 1];
     if (pb == pe) {
@@ -1195,78 +1197,97 @@ __global__ __launch_bounds__(64) void k_cull_site(ClipArgs a, int depth, unsigne
       const int z = nbr[v];
       if ((unsigned)z >= (unsigned)a.n_sites) {
         eb[v] = make_float4(0.f, 0.f, 0.f, 0.f);
-        db[v] = kInvalidNbr;
+        ec[v] = make_float4(kInvalidNbr, 0.f, 0.f, 0.f);
         continue;
       }
       const P4 zp = site_at(a, z);
       const P4 eq = bisector(sp, zp);
       const P4 d = sub3(zp, sp);
+      const float dd = dot3(d, d);
       eb[v] = make_float4(eq.x, eq.y, eq.z, eq.w);
-      db[v] = dot3(d, d);
+      ec[v] = make_float4(dd, fabsf(eq.x) + fabsf(eq.y) + fabsf(eq.z), fabsf(eq.w), kCullMargin * __builtin_amdgcn_sqrtf(dd));
     }
     __syncthreads();
-    unsigned w = pb, n_culled = 0;
-    for (unsigned c0 = pb; c0 < pe; c0 += 64) {
-      const unsigned i = c0 + lane;
-      const bool live = i < pe;
-      const unsigned pair = live ? pair_ids[i] : 0u;
-      P4 v[4] = {};
-      float pmax = 0.f, vmin2 = 0.f;
-      if (live) {
-        tet_corners(a, (int)(pair / (unsigned)a.TK), v);
-        vmin2 = INFINITY;
+    unsigned w = pb;
+    // two pairs per lane per trip (c0 + lane and c0 + 64 + lane), so that
+    // their loads are in flight together
+    for (unsigned c0 = pb; c0 < pe; c0 += 128) {
+      unsigned pair[2];
+      bool live[2], culled[2] = {false, false}, open[2];
+      P4 v[2][4] = {};
+      float pmax[2] = {0.f, 0.f}, vmin2[2] = {0.f, 0.f};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          pmax = fmaxf(pmax, fmaxf(fabsf(v[j].x), fmaxf(fabsf(v[j].y), fabsf(v[j].z))));
-          const P4 d = sub3(v[j], sp);
-          vmin2 = fminf(vmin2, dot3(d, d));
-        }
+      for (int h = 0; h < 2; ++h) {
+        const unsigned i = c0 + 64 * h + lane;
+        live[h] = i < pe;
+        pair[h] = live[h] ? pair_ids[i] : 0u;
       }
-      bool culled = false, open = live;
-      for (int q = 0; q < depth && open; ++q) {
-        const float dd = db[q];
-        if (dd == kInvalidNbr || 4.f * vmin2 <= dd) {
-          // an invalid neighbour ends the clipping there; and once the tet's
-          // nearest corner is within |z - s| / 2 of the seed it is on the
-          // seed's side of this bisector and of every later one (the
-          // neighbour distances do not decrease)
-          open = false;
-        } else {
-          const float4 e = eb[q];
-          // rounding bound of the plane evaluation (relative kCullRound of
-          // |n|_1 * max|x| + |w|) plus the required depth beyond the plane
-          // (kCullMargin in distance, times |n|)
-          const float lim = -fmaf(kCullRound, fmaf(fabsf(e.x) + fabsf(e.y) + fabsf(e.z), pmax, fabsf(e.w)),
-                                  kCullMargin * __builtin_amdgcn_sqrtf(dd));
-          bool all = true;
 #pragma unroll
-          for (int k = 0; k < 4; ++k) all = all && (fmaf(e.z, v[k].z, fmaf(e.y, v[k].y, e.x * v[k].x)) + e.w < lim);
-          if (all) {
-            culled = true;
-            open = false;
+      for (int h = 0; h < 2; ++h)
+        if (live[h]) tet_corners(a, (int)(pair[h] / (unsigned)a.TK), v[h]);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        open[h] = live[h];
+        if (live[h]) {
+          vmin2[h] = INFINITY;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            pmax[h] = fmaxf(pmax[h], fmaxf(fabsf(v[h][j].x), fmaxf(fabsf(v[h][j].y), fabsf(v[h][j].z))));
+            const P4 d = sub3(v[h][j], sp);
+            vmin2[h] = fminf(vmin2[h], dot3(d, d));
           }
         }
       }
-      if (culled && a.pair_status) a.pair_status[pair] = (uint8_t)kNoIntersection;
-      if (culled && a.pair_vals) {
-        double* pvals = a.pair_vals + 4 * (size_t)pair;
-        pvals[0] = pvals[1] = pvals[2] = pvals[3] = 0.0;
+      for (int q = 0; q < depth && (open[0] || open[1]); ++q) {
+        const float4 f = ec[q];
+        const float4 e = eb[q];
+        // rounding bound of the plane evaluation (relative kCullRound of
+        // |n|_1 * max|x| + |w|: three roundings of partial sums bounded by
+        // that) plus the required depth beyond the plane (kCullMargin in
+        // distance, times |n|)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          if (!open[h]) continue;
+          if (f.x == kInvalidNbr || 4.f * vmin2[h] <= f.x) {
+            // an invalid neighbour ends the clipping there; and once the
+            // tet's nearest corner is within |z - s| / 2 of the seed it is on
+            // the seed's side of this bisector and of every later one (the
+            // neighbour distances do not decrease)
+            open[h] = false;
+            continue;
+          }
+          const float lim = -fmaf(kCullRound, fmaf(f.y, pmax[h], f.z), f.w);
+          float hi = -INFINITY;
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            hi = fmaxf(hi, fmaf(e.z, v[h][k].z, fmaf(e.y, v[h][k].y, fmaf(e.x, v[h][k].x, e.w))));
+          if (hi < lim) {
+            culled[h] = true;
+            open[h] = false;
+          }
+        }
       }
-      n_culled += (unsigned)__popcll(__ballot(culled));
-      // survivors to the front: every position written here has been read
-      // (w never passes the chunk being read)
-      const bool keep = live && !culled;
-      const unsigned long long mask = __ballot(keep);
-      const unsigned below =
-          (unsigned)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
-      if (keep) pair_ids[w + below] = pair;
-      w += (unsigned)__popcll(mask);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (culled[h] && a.pair_status) a.pair_status[pair[h]] = (uint8_t)kNoIntersection;
+        if (culled[h] && a.pair_vals) {
+          double* pvals = a.pair_vals + 4 * (size_t)pair[h];
+          pvals[0] = pvals[1] = pvals[2] = pvals[3] = 0.0;
+        }
+        n_culled += (unsigned)__popcll(__ballot(culled[h]));
+        // survivors to the front, in order: every position written here has
+        // been read (w never passes the pairs read so far)
+        const bool keep = live[h] && !culled[h];
+        const unsigned long long mask = __ballot(keep);
+        const unsigned below =
+            (unsigned)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+        if (keep) pair_ids[w + below] = pair[h];
+        w += (unsigned)__popcll(mask);
+      }
     }
-    if (lane == 0) {
-      pair_count[s] = w - pb;
-      if (a.status_hist && n_culled) atomicAdd(a.status_hist + kNoIntersection, (unsigned long long)n_culled);
-    }
+    if (lane == 0) pair_count[s] = w - pb;
   }
+  if (lane == 0 && a.status_hist && n_culled) atomicAdd(a.status_hist + kNoIntersection, (unsigned long long)n_culled);
 }
 
 // Volume pass of the site-major form: one lane per record.  The planes are
@@ -1278,6 +1299,63 @@ __global__ __launch_bounds__(64) void k_cull_site(ClipArgs a, int depth, unsigne
 // security_radius_not_reached to the deeper-list re-clip when they are on,
 // anything else is accumulated.
 constexpr int kMomentsBlock = 256;
+
+// Sum over the lanes of mask m (wave-uniform) of a per-lane value, in every
+// lane; 64-bit integers add exactly, so the order does not matter.
+__device__ __forceinline__ long long wave_sum_ll(long long x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+__device__ __forceinline__ float wave_sum_f(float x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
+// Accumulation of a wave's pairs: lanes with acc set add S to site s.  Lanes
+// of one site are summed across the wave first (a wave's records mostly
+// belong to one or two sites), so each site gets one set of atomics per wave
+// instead of one per lane.  The FP64 mode's per-pair values are rounded to
+// the fixed point first, as accumulate<1> does, and integer sums are exact,
+// so the site sums are the same bit for bit; the FP32 mode's order of
+// additions varies anyway.
+template <int ACC>
+__device__ __forceinline__ void accumulate_wave(const ClipArgs& a, bool acc, int s, const PairSums<ACC>& S) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(acc);
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const int ks = __shfl(s, leader);
+    const bool mine = acc && s == ks && ((todo >> lane) & 1ull);
+    todo &= ~__ballot(mine);
+    if constexpr (ACC == 1) {
+      const long long v = wave_sum_ll(mine ? __double2ll_rn(S.v * kFixScale) : 0ll);
+      const long long x = wave_sum_ll(mine ? __double2ll_rn(S.x * kFixScale) : 0ll);
+      const long long y = wave_sum_ll(mine ? __double2ll_rn(S.y * kFixScale) : 0ll);
+      const long long z = wave_sum_ll(mine ? __double2ll_rn(S.z * kFixScale) : 0ll);
+      if (lane == leader) {
+        unsigned long long* p = a.acc_fix + 4 * (size_t)ks;
+        atomicAdd(p + 0, (unsigned long long)v);
+        atomicAdd(p + 1, (unsigned long long)x);
+        atomicAdd(p + 2, (unsigned long long)y);
+        atomicAdd(p + 3, (unsigned long long)z);
+      }
+    } else {
+      const float v = wave_sum_f(mine ? S.v : 0.f);
+      const float x = wave_sum_f(mine ? S.x : 0.f);
+      const float y = wave_sum_f(mine ? S.y : 0.f);
+      const float z = wave_sum_f(mine ? S.z : 0.f);
+      if (lane == leader) {
+        float* p = a.acc_f32 + 4 * (size_t)ks;
+        unsafeAtomicAdd(p + 0, v);
+        unsafeAtomicAdd(p + 1, x);
+        unsafeAtomicAdd(p + 2, y);
+        unsafeAtomicAdd(p + 3, z);
+      }
+    }
+  }
+}
 
 template <int MF, int ACC>
 __global__ __launch_bounds__(kMomentsBlock) void k_site_moments(ClipArgs a) {
@@ -1294,53 +1372,60 @@ __global__ __launch_bounds__(kMomentsBlock) void k_site_moments(ClipArgs a) {
   c.keep_srnr = false;
   const unsigned n = min(*a.rec_count, a.rec_cap);
   const int stride = rec_stride(a.tcap);
-  for (unsigned i = blockIdx.x * kMomentsBlock + threadIdx.x; i < n; i += gridDim.x * kMomentsBlock) {
-    const uint4 m = a.rec_meta[i];
-    if (m.x == ~0u) continue;  // a reserved slot left unused
+  // wave-uniform loop: every lane takes part in the accumulation's shuffles
+  for (unsigned base = blockIdx.x * kMomentsBlock + (threadIdx.x & ~63u); base < n;
+       base += gridDim.x * kMomentsBlock) {
+    const unsigned i = base + lane;
+    uint4 m = make_uint4(~0u, 0u, 0u, 0u);
+    if (i < n) m = a.rec_meta[i];
+    const bool rec = m.x != ~0u;  // a reserved slot left unused, or past the end
     const unsigned pair = m.x;
-    const int s = (int)m.y;
-    P4 v[4];
-    tet_corners(a, (int)m.z, v);
-    c.seed = site_at(a, s);
-    c.ps.seed = c.seed;
-    c.ps.nbr = a.site_knn + (size_t)s * a.K;
-    c.put_tet_faces(v);
-    c.ts.tr = a.rec_tris + (size_t)i * stride;
-    c.nt = (int)(m.w & 255u);
-    int status = (int)(m.w >> 8);
+    const int s = rec ? (int)m.y : 0;
     PairSums<ACC> S;
-    if (MF == 0)
-      c.template moments_ref<ACC>(S);
-    else
-      c.template moments_robust<ACC>(S);
+    int status = kNoIntersection;
     bool acc = false;
-    if (!(isfinite((double)S.v) && isfinite((double)S.x) && isfinite((double)S.y) && isfinite((double)S.z)))
-      status = kNeedsExactPredicates;
-    else if (fabs((double)S.v) < 0.1)
-      status = kNoIntersection;
-    else
-      acc = true;
-    // then as the site-major kernel's finish step
-    if (status == kTriangleOverflow && a.retry_list && a.tcap < a.tcap_full) {
-      const unsigned slot = atomicAdd(a.retry_count, 1u);
-      if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = pair;
-      continue;
+    if (rec) {
+      P4 v[4];
+      tet_corners(a, (int)m.z, v);
+      c.seed = site_at(a, s);
+      c.ps.seed = c.seed;
+      c.ps.nbr = a.site_knn + (size_t)s * a.K;
+      c.put_tet_faces(v);
+      c.ts.tr = a.rec_tris + (size_t)i * stride;
+      c.nt = (int)(m.w & 255u);
+      status = (int)(m.w >> 8);
+      if (MF == 0)
+        c.template moments_ref<ACC>(S);
+      else
+        c.template moments_robust<ACC>(S);
+      if (!(isfinite((double)S.v) && isfinite((double)S.x) && isfinite((double)S.y) && isfinite((double)S.z)))
+        status = kNeedsExactPredicates;
+      else if (fabs((double)S.v) < 0.1)
+        status = kNoIntersection;
+      else
+        acc = true;
+      // then as the site-major kernel's finish step
+      if (status == kTriangleOverflow && a.retry_list && a.tcap < a.tcap_full) {
+        const unsigned slot = atomicAdd(a.retry_count, 1u);
+        if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = pair;
+        acc = false;
+      } else if (status == kSecurityRadiusNotReached && a.srnr_list) {
+        const unsigned slot = atomicAdd(a.srnr_count, 1u);
+        if (slot < (unsigned)a.n_pairs) a.srnr_list[slot] = pair;
+        acc = false;
+      } else {
+        if (a.pair_status) a.pair_status[pair] = (uint8_t)status;
+        if (a.pair_vals) {
+          double* pvals = a.pair_vals + 4 * (size_t)pair;
+          pvals[0] = acc ? (double)S.v : 0.0;
+          pvals[1] = acc ? (double)S.x : 0.0;
+          pvals[2] = acc ? (double)S.y : 0.0;
+          pvals[3] = acc ? (double)S.z : 0.0;
+        }
+        atomicAdd(&hist[status], 1u);
+      }
     }
-    if (status == kSecurityRadiusNotReached && a.srnr_list) {
-      const unsigned slot = atomicAdd(a.srnr_count, 1u);
-      if (slot < (unsigned)a.n_pairs) a.srnr_list[slot] = pair;
-      continue;
-    }
-    if (acc) accumulate<ACC>(a, s, S);
-    if (a.pair_status) a.pair_status[pair] = (uint8_t)status;
-    if (a.pair_vals) {
-      double* pvals = a.pair_vals + 4 * (size_t)pair;
-      pvals[0] = acc ? (double)S.v : 0.0;
-      pvals[1] = acc ? (double)S.x : 0.0;
-      pvals[2] = acc ? (double)S.y : 0.0;
-      pvals[3] = acc ? (double)S.z : 0.0;
-    }
-    atomicAdd(&hist[status], 1u);
+    accumulate_wave<ACC>(a, acc, s, S);
   }
   __syncthreads();
   if (a.status_hist && threadIdx.x < 7 && hist[threadIdx.x])
@@ -1804,8 +1889,8 @@ hipError_t cull_sites(const ClipArgs& a, unsigned* pair_ids, unsigned* pair_coun
   const int K = a.site_depth > 0 ? std::min(a.site_depth, a.K) : a.K;
   const int depth = std::min(a.cull_depth, K);
   if (depth <= 0 || depth > kCullMaxDepth) return hipErrorInvalidValue;
-  const unsigned blocks = (unsigned)std::min(a.n_sites, 65536 * 4);
-  hipLaunchKernelGGL(k_cull_site, dim3(blocks), dim3(64), 0, st, a, depth, pair_ids, pair_count);
+  const unsigned blocks = (unsigned)std::min(a.n_sites, 16384);
+  hipLaunchKernelGGL(k_cull_site, dim3(blocks), dim3(64), 32 * (size_t)depth, st, a, depth, pair_ids, pair_count);
   return hipGetLastError();
 }
 
