@@ -288,14 +288,15 @@ def main():
         fpp = flops_per_pair(cpu["events"])
         log(f"cpu baseline: {cpu['iters_per_s']:.4f} iters/s on {cpu['cores']} threads; {fpp:.0f} FLOP/pair")
 
-    # the site-major clipping kernel and its volume pass: the clip phase minus
-    # the re-clipping of pairs that overflowed its caps (HIP events on the
-    # engine's stream, around the two launches)
+    # the site-major clipping with its emptiness pass and volume pass: the
+    # clip phase minus the re-clipping of pairs that overflowed its caps (HIP
+    # events on the engine's stream, around the three launches)
     kern_s = (ts["clip_ms"] - ts["overflow_ms"]) / n / 1e3
     roof = None
     if fpp is not None:
         achieved = fpp * n_pairs / kern_s / 1e12
-        roof = {"bound": "valu", "kernel": "k_clip_site + k_site_moments (clipping; volumes, moments, accumulation)", "achieved": round(achieved, 3),
+        roof = {"bound": "valu", "kernel": "k_cull_site + k_clip_site + k_site_moments (emptiness test; clipping; volumes, "
+                                             "moments, accumulation)", "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
                 "traffic": profiled_traffic(), "flop_per_launch": fpp * n_pairs, "kernel_ms": round(kern_s * 1e3, 3)}
     value = args.steps / wall
