@@ -308,24 +308,24 @@ constexpr int kQueryWaves = 4;
 // staged, and the cell walk stops there.
 constexpr int kQueryRun = 8;
 
-template <int M>
-__global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
-  __shared__ uint64_t stage_all[kQueryWaves][128];
-  const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
-  uint64_t* stage = stage_all[wv];
-  const int q0 = (blockIdx.x * kQueryWaves + wv) * kQueryRun;
-  int nq = a.n_query;
-  if (a.n_query_dev) nq = min(nq, (int)min(*a.n_query_dev, (unsigned)a.n_query));
-  float prev_r = -1.f;
-  float4 prev_qp = {0.f, 0.f, 0.f, 0.f};
-  int prev_inst = -1;
-  for (int q = q0; q < q0 + kQueryRun && q < nq; ++q) {  // wave-uniform
+// After the first query of a run, the points that the run's other queries can
+// need are gathered once into LDS (at most kUnionCap): query i's k-th distance
+// is at most r0 + |q_i - q_0| (the first query's k neighbours, with q_0 in
+// place of q_i itself where the query excludes itself), so every point such a
+// query can keep lies within R = max_i(|q_i - c| + r0 + |q_i - q_0|) of the
+// centre c of the run.  Those queries scan the list instead of walking the
+// grid, with the same keys and the same bounds, so their results are the
+// same; a run whose list would not fit walks as before.
+#ifndef VD_KNN_UNION_CAP
+#define VD_KNN_UNION_CAP 384
+#endif
+constexpr int kUnionCap = VD_KNN_UNION_CAP;
 
-  float4 qp;
-  int self = -1;
-  int out_row = q;
-  int inst = 0;
+// Point of query q, its excluded reference id (or -1), output row and instance.
+__device__ __forceinline__ void query_point(const QueryArgs& a, int q, float4& qp, int& self, int& out_row, int& inst) {
+  self = -1;
+  out_row = q;
+  inst = 0;
   if (a.mode == 0) {
     qp = a.sorted_pts[q];
     self = a.sorted_ids[q];
@@ -335,22 +335,47 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
     self = a.q_ids[q];
     qp = a.qpts[self];
     if (a.pt_inst) inst = a.pt_inst[self];
-  } else if (a.q_inst) {
-    inst = a.q_inst[q];
+  } else {
+    if (a.q_inst) inst = a.q_inst[q];
+    if (a.mode == 1) {
+      const int4 t = a.tets[q];
+      const float4 v0 = a.verts[t.x], v1 = a.verts[t.y], v2 = a.verts[t.z], v3 = a.verts[t.w];
+      qp.x = (((v0.x + v1.x) + v2.x) + v3.x) * 0.25f;
+      qp.y = (((v0.y + v1.y) + v2.y) + v3.y) * 0.25f;
+      qp.z = (((v0.z + v1.z) + v2.z) + v3.z) * 0.25f;
+      qp.w = 0.f;
+    } else {
+      qp = a.qpts[q];
+    }
   }
+}
+
+template <int M>
+__global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
+  __shared__ uint64_t stage_all[kQueryWaves][128];
+  __shared__ float4 uni_all[kQueryWaves][kUnionCap];  // (x, y, z, id bits)
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  uint64_t* stage = stage_all[wv];
+  float4* uni = uni_all[wv];
+  const int q0 = (blockIdx.x * kQueryWaves + wv) * kQueryRun;
+  int nq = a.n_query;
+  if (a.n_query_dev) nq = min(nq, (int)min(*a.n_query_dev, (unsigned)a.n_query));
+  float prev_r = -1.f;
+  float4 prev_qp = {0.f, 0.f, 0.f, 0.f};
+  int prev_inst = -1;
+  // the run's point list: n_uni points (-1: none), valid for instance uni_inst
+  int n_uni = -1, uni_inst = -1;
+  float r0 = 0.f;
+  float4 q0p = {0.f, 0.f, 0.f, 0.f};
+  for (int q = q0; q < q0 + kQueryRun && q < nq; ++q) {  // wave-uniform
+
+  float4 qp;
+  int self, out_row, inst;
+  query_point(a, q, qp, self, out_row, inst);
   const GridShape g = a.grid[inst];
   const int cbase = a.cell_base ? a.cell_base[inst] : 0;
-  if (a.mode == 0) {
-  } else if (a.mode == 1) {
-    const int4 t = a.tets[q];
-    const float4 v0 = a.verts[t.x], v1 = a.verts[t.y], v2 = a.verts[t.z], v3 = a.verts[t.w];
-    qp.x = (((v0.x + v1.x) + v2.x) + v3.x) * 0.25f;
-    qp.y = (((v0.y + v1.y) + v2.y) + v3.y) * 0.25f;
-    qp.z = (((v0.z + v1.z) + v2.z) + v3.z) * 0.25f;
-    qp.w = 0.f;
-  } else if (a.mode == 2) {
-    qp = a.qpts[q];
-  }
+  const bool from_list = n_uni >= 0 && inst == uni_inst;
   const int qc[3] = {cell_coord(qp.x, g.lo[0], g.inv_h, g.dim[0]), cell_coord(qp.y, g.lo[1], g.inv_h, g.dim[1]),
                      cell_coord(qp.z, g.lo[2], g.inv_h, g.dim[2])};
 
@@ -363,6 +388,13 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
     const float b = prev_r + sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
     const float b2 = b * b * (1.f + 1e-5f);
     if (b2 < INFINITY) bound = ((uint64_t)__float_as_uint(b2) << 32) | 0xffffffffull;
+  }
+  if (from_list) {
+    // the bound the list was gathered for: r0 + |q - q0|
+    const float dx = qp.x - q0p.x, dy = qp.y - q0p.y, dz = qp.z - q0p.z;
+    const float b = r0 + sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+    const float b2 = b * b * (1.f + 1e-5f);
+    bound = umin64(bound, ((uint64_t)__float_as_uint(b2) << 32) | 0xffffffffull);
   }
   uint64_t thr = bound;
   int staged = 0;
@@ -386,6 +418,21 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
     }
   };
 
+  if (from_list) {
+    for (int f0 = 0; f0 < n_uni; f0 += 64) {
+      const int f = f0 + lane;
+      uint64_t key = ~0ull;
+      bool ok = false;
+      if (f < n_uni) {
+        const float4 p = uni[f];
+        const uint32_t id = __float_as_uint(p.w);
+        key = knn_key(dist2(p, qp), id);
+        ok = ((int)id != self) && key < thr;
+      }
+      push(key, ok);
+    }
+    done = true;
+  }
   for (int ob = 0; ob < a.n_offsets && !done; ob += 64) {
     const float kth = key_d2(thr);
     // offsets are sorted by lb: once the first of the batch is beyond the k-th
@@ -481,6 +528,107 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
   prev_r = kth == ~0ull ? -1.f : sqrtf(key_d2(kth));
   prev_qp = qp;
   prev_inst = inst;
+
+  const int q_end = min(q0 + kQueryRun, nq);
+  if (q == q0 && prev_r >= 0.f && q + 1 < q_end) {
+    // the run's other queries: their points (lane j takes query q0 + j)
+    float4 pj = qp;
+    int sj, rj, ij = inst;
+    const int qj = q0 + lane;
+    if (lane >= 1 && qj < q_end) query_point(a, qj, pj, sj, rj, ij);
+    const bool in = lane >= 1 && qj < q_end;
+    const bool same = __ballot(in && ij != inst) == 0;
+    if (same) {
+      const int n_in = q_end - q0 - 1;
+      float cx = in ? pj.x : 0.f, cy = in ? pj.y : 0.f, cz = in ? pj.z : 0.f;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        cx += __shfl_xor(cx, o);
+        cy += __shfl_xor(cy, o);
+        cz += __shfl_xor(cz, o);
+      }
+      const float inv = 1.f / (float)n_in;
+      const float4 c = make_float4(cx * inv, cy * inv, cz * inv, 0.f);
+      float rj_need = 0.f;
+      if (in) {
+        const float ax = pj.x - c.x, ay = pj.y - c.y, az = pj.z - c.z;
+        const float bx = pj.x - qp.x, by = pj.y - qp.y, bz = pj.z - qp.z;
+        rj_need = sqrtf(fmaf(az, az, fmaf(ay, ay, ax * ax))) + prev_r + sqrtf(fmaf(bz, bz, fmaf(by, by, bx * bx)));
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) rj_need = fmaxf(rj_need, __shfl_xor(rj_need, o));
+      // rounding margins of the FP32 distances on either side
+      const float R = rj_need * (1.f + 1e-4f) + 1e-3f * g.h;
+      const float R2 = R * R;
+      const int cc[3] = {cell_coord(c.x, g.lo[0], g.inv_h, g.dim[0]), cell_coord(c.y, g.lo[1], g.inv_h, g.dim[1]),
+                         cell_coord(c.z, g.lo[2], g.inv_h, g.dim[2])};
+      int n = 0;
+      bool fits = true, covered = false;
+      for (int ob = 0; ob < a.n_offsets && fits; ob += 64) {
+        const float lbd = fmaxf(0.f, sqrtf((float)a.offsets[ob].w) * g.h - 2.f * g.slack);
+        if (lbd * lbd * (1.f - 1e-5f) > R2) {
+          covered = true;  // offsets are sorted by lb: the rest are beyond R too
+          break;
+        }
+        int cnt = 0, st = 0;
+        const int oi = ob + lane;
+        if (oi < a.n_offsets) {
+          const int4 o = a.offsets[oi];
+          const int ox = cc[0] + o.x, oy = cc[1] + o.y, oz = cc[2] + o.z;
+          if (ox >= 0 && ox < g.dim[0] && oy >= 0 && oy < g.dim[1] && oz >= 0 && oz < g.dim[2] &&
+              !(cell_bound(g, c, ox, oy, oz) > R2)) {
+            const int cell = cbase + ox + g.dim[0] * (oy + g.dim[1] * oz);
+            st = a.start[cell];
+            cnt = a.start[cell + 1] - st;
+          }
+        }
+        int incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+          const int v = __shfl_up(incl, d);
+          if (lane >= d) incl += v;
+        }
+        const int excl = incl - cnt;
+        const int total = __shfl(incl, 63);
+        for (int f0 = 0; f0 < total && fits; f0 += 64) {
+          const int f = f0 + lane;
+          int own = 0;
+#pragma unroll
+          for (int step = 32; step > 0; step >>= 1) {
+            const int cand = own + step;
+            const int pe = __shfl(excl, cand & 63);
+            if (cand < 64 && pe <= f) own = cand;
+          }
+          const int o_st = __shfl(st, own), o_ex = __shfl(excl, own);
+          bool keep = false;
+          float4 p = {0.f, 0.f, 0.f, 0.f};
+          if (f < total) {
+            const int idx = o_st + (f - o_ex);
+            p = a.sorted_pts[idx];
+            p.w = __uint_as_float((uint32_t)a.sorted_ids[idx]);
+            keep = !(dist2(p, c) > R2);
+          }
+          const uint64_t m = __ballot(keep);
+          const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+          const int add = __popcll(m);
+          if (n + add > kUnionCap) {
+            fits = false;
+          } else {
+            if (keep) uni[n + below] = p;
+            n += add;
+          }
+        }
+      }
+      // beyond the offset table the walk scans the whole instance: no list
+      if (fits && covered) {
+        wave_sync();
+        n_uni = n;
+        uni_inst = inst;
+        r0 = prev_r;
+        q0p = qp;
+      }
+    }
+  }
   }  // query loop
 }
 
