@@ -73,6 +73,10 @@ struct PlaneStoreShared {
     return {v.x, v.y, v.z, v.w};
   }
   __device__ __forceinline__ void put(int i, const P4& p) { face[i * 64 + lane] = make_float4(p.x, p.y, p.z, p.w); }
+  __device__ __forceinline__ void bind(float4* f, int l) {
+    face = f;
+    lane = l;
+  }
 };
 
 // Kernel C: the stored cell's planes (ids 0..nvc-1) in a table shared by the
@@ -895,9 +899,8 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   uint2* queue = (uint2*)(smem + off[5]);
   unsigned* hist = (unsigned*)(smem + off[6]);
   Cell<PlaneStoreShared, TriCached> c;
-  c.ps.face = (float4*)(smem + off[3]);
+  c.ps.bind((float4*)(smem + off[3]), lane);
   c.ps.bis = bis;
-  c.ps.lane = lane;
   c.ts.tr = (uint2*)(smem + off[4]);
   c.ts.lane = lane;
   c.nx = smem + off[7];

@@ -48,8 +48,12 @@ struct DevBuf {
   }
 };
 
-// Bisectors per site table of the site-major launch for K neighbours.
-int site_depth_of(const vd_config& c, int K) { return c.site_depth > 0 ? std::min(c.site_depth, K) : K; }
+// Bisectors per site table of the site-major launch for K neighbours: by
+// default 40, which keeps the kernel's LDS at 9 waves per CU (the kernel is
+// latency bound and scales with them); the few pairs that need more are
+// recomputed at the full depth.
+constexpr int kAutoSiteDepth = 40;
+int site_depth_of(const vd_config& c, int K) { return std::min(c.site_depth > 0 ? c.site_depth : kAutoSiteDepth, K); }
 
 int tet_k_heuristic(int n_site, int n_tet, int preferred) {
   // voronoi.cu:936

@@ -94,7 +94,7 @@ typedef struct vd_config {
                          0 = automatic (n_pairs / 10 + 262144), -1 = every volume in place */
   int site_depth;     /* site-major form: bisectors per site table of the launch; a pair that
                          needs more is recomputed from scratch at site_k by the retry pass,
-                         so results do not depend on it; 0 = automatic */
+                         so results do not depend on it; 0 = automatic (40) */
 } vd_config;
 
 typedef struct vd_timing {
