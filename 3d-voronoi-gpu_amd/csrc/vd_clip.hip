@@ -45,6 +45,13 @@ __device__ __forceinline__ int tri_at(uint32_t t, int k) { return (int)((t >> (8
 #endif
 constexpr int kFarUnroll = VD_FAR_UNROLL, kCandUnroll = VD_CAND_UNROLL;
 
+// Bisectors a lane of the cached clipping loops may test per trip before the
+// trip's cuts run.
+#ifndef VD_TESTS_PER_TRIP
+#define VD_TESTS_PER_TRIP 3
+#endif
+constexpr int kTestsPerTrip = VD_TESTS_PER_TRIP;
+
 // Storage policies --------------------------------------------------------
 //
 // PlaneStoreLocal: planes [pcap][64] per lane, ids are positions (reference).
@@ -693,22 +700,73 @@ __device__ __forceinline__ bool run_pair(const ClipArgs& a, Cell<PS, TS>& c, con
                                          PairSums<ACC>& S) {
   const int K = a.K;
   bool stopped = false;
-  for (int v = 0; v < K; ++v) {
-    const int z = nbr[v];
-    if ((unsigned)z >= (unsigned)a.n_sites) {  // defensive: never read outside the site array
-      c.status = kInconsistentBoundary;
-      return false;
+  if constexpr (TS::kCached) {
+    // the cached form: up to kTestsPerTrip bisectors tested per loop trip,
+    // until one removes a vertex, then its cut, so that the lanes of a wave
+    // run their cuts together (the same operations in the same order per lane)
+    int v = 0;
+    bool go = K > 0;
+    while (go) {
+      bool cut = false, ksec = false;
+      uint32_t conf = 0;
+      float cut_d2z = 0.f;
+      int cut_pid = 0;
+      for (int it = 0; it < kTestsPerTrip && go; ++it) {
+        const int z = nbr[v];
+        if ((unsigned)z >= (unsigned)a.n_sites) {  // defensive: never read outside the site array
+          c.status = kInconsistentBoundary;
+          return false;
+        }
+        int pid;
+        P4 eq;
+        float d2z;
+        plane_of(v, z, pid, eq, d2z);
+        const int r = c.clip_test(eq, d2z, conf, ksec);
+        if (r == 2) {
+          cut = true;
+          cut_d2z = d2z;
+          cut_pid = pid;
+          break;
+        }
+        if (r >= 0 ? r != 0 : c.security_reached(d2z)) {
+          stopped = true;
+          go = false;
+        } else if (c.status != kSuccess) {
+          return false;
+        } else if (++v == K) {
+          go = false;
+        }
+      }
+      if (cut) {
+        const int r = c.clip_cut(cut_pid, conf, ksec, cut_d2z);
+        if (r >= 0 ? r != 0 : c.security_reached(cut_d2z)) {
+          stopped = true;
+          go = false;
+        } else if (c.status != kSuccess) {
+          return false;
+        } else if (++v == K) {
+          go = false;
+        }
+      }
     }
-    int pid;
-    P4 eq;
-    float d2z;
-    plane_of(v, z, pid, eq, d2z);
-    const int sec = c.clip(pid, eq, d2z);
-    if (sec >= 0 ? sec != 0 : c.security_reached(d2z)) {
-      stopped = true;
-      break;
+  } else {
+    for (int v = 0; v < K; ++v) {
+      const int z = nbr[v];
+      if ((unsigned)z >= (unsigned)a.n_sites) {  // defensive: never read outside the site array
+        c.status = kInconsistentBoundary;
+        return false;
+      }
+      int pid;
+      P4 eq;
+      float d2z;
+      plane_of(v, z, pid, eq, d2z);
+      const int sec = c.clip(pid, eq, d2z);
+      if (sec >= 0 ? sec != 0 : c.security_reached(d2z)) {
+        stopped = true;
+        break;
+      }
+      if (c.status != kSuccess) return false;
     }
-    if (c.status != kSuccess) return false;
   }
   if (!stopped || K == 0) {
     // the last neighbour's test, as the reference makes it after the loop
@@ -846,11 +904,6 @@ __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
 constexpr int kQueue = 128;           // ring of queued (pair, slot) entries
 constexpr float kInvalidNbr = -1.f;   // table mark of an out-of-range neighbour id
 constexpr int kIdle = 0, kClipping = 1;  // lane states
-// bisectors a lane may test per trip before the trip's cuts run
-#ifndef VD_TESTS_PER_TRIP
-#define VD_TESTS_PER_TRIP 3
-#endif
-constexpr int kTestsPerTrip = VD_TESTS_PER_TRIP;
 constexpr int kRecChunk = 128;           // record slots a wave reserves at a time
 // Records of polytopes with more than kRecSplit vertices go to chunks of their
 // own, so that a wave of the volume pass (64 records of one chunk) runs over
