@@ -227,10 +227,8 @@ struct TopK {
   uint64_t s[M];
 };
 
-// Sorted merge of 64 new keys (one per lane) into the top list.
-template <int M>
-__device__ __forceinline__ void topk_merge(TopK<M>& L, uint64_t x, int lane) {
-  // bitonic sort of x across the wave, ascending
+// Bitonic sort of one key per lane across the wave, ascending.
+__device__ __forceinline__ uint64_t wave_sort64(uint64_t x, int lane) {
 #pragma unroll
   for (int k = 2; k <= 64; k <<= 1) {
 #pragma unroll
@@ -240,6 +238,18 @@ __device__ __forceinline__ void topk_merge(TopK<M>& L, uint64_t x, int lane) {
       const bool lower = (lane & j) == 0;
       x = (lower == up) ? umin64(x, o) : umax64(x, o);
     }
+  }
+  return x;
+}
+
+// Sorted merge of 64 new keys (one per lane) into the top list; into an
+// empty list (first) the sorted keys are the list.
+template <int M>
+__device__ __forceinline__ void topk_merge(TopK<M>& L, uint64_t x, int lane, bool first = false) {
+  x = wave_sort64(x, lane);
+  if (first) {
+    L.s[0] = x;
+    return;
   }
   // min against the reversed batch gives the 64*M smallest as a bitonic sequence
   const uint64_t r = bperm64(x, 63 - lane);
@@ -399,6 +409,7 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
   uint64_t thr = bound;
   int staged = 0;
   bool done = false;
+  bool empty = true;  // no key merged yet (wave-uniform)
 
   auto push = [&](uint64_t key, bool ok) {
     const uint64_t mask = __ballot(ok);
@@ -413,7 +424,8 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
       if (lane + 64 < staged) stage[lane] = rest;
       staged -= 64;
       wave_sync();
-      topk_merge<M>(L, x, lane);
+      topk_merge<M>(L, x, lane, empty);
+      empty = false;
       thr = umin64(bound, topk_kth<M>(L, a.k));
     }
   };
@@ -511,7 +523,7 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
   }
   if (staged > 0) {
     const uint64_t x = lane < staged ? stage[lane] : ~0ull;
-    topk_merge<M>(L, x, lane);
+    topk_merge<M>(L, x, lane, empty);
   }
 #pragma unroll
   for (int s = 0; s < M; ++s) {
