@@ -901,7 +901,7 @@ __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
 // (in place, a lane would either stall the wave's clipping or wait idle for
 // others to finish).
 
-constexpr int kQueue = 128;           // ring of queued (pair, slot) entries
+constexpr int kQueue = 128;           // ring of queued pair ids
 constexpr float kInvalidNbr = -1.f;   // table mark of an out-of-range neighbour id
 constexpr int kIdle = 0, kClipping = 1;  // lane states
 constexpr int kRecChunk = 128;           // record slots a wave reserves at a time
@@ -927,10 +927,10 @@ __host__ __device__ inline size_t site_lds_layout(int K, int tcap, size_t* off) 
   size_t o = 0;
   off[0] = o; o += 16 * 2 * K4;        // bis   float4 [2][K4]
   off[1] = o;                          // (unused)
-  off[2] = o; o += 4 * 2 * K4;         // bd2   float  [2][K4]
+  off[2] = o;                          // (unused)
   off[3] = o; o += 16 * 4 * 64;        // face float4 [4][64]
   off[4] = o; o += 8 * (size_t)tcap * 64;  // tr uint2 [tcap][64]
-  off[5] = o; o += 8 * kQueue;         // queue int2 [kQueue]
+  off[5] = o; o += 4 * kQueue;         // queue u32 [kQueue]
   off[6] = o; o += 4 * 8;              // hist  unsigned [8]
   off[7] = o; o += (size_t)(4 + K) * 64;  // nx u8 [4+K][64]
   return (o + 15) & ~(size_t)15;
@@ -948,8 +948,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
   size_t off[8];
   site_lds_layout(K, a.tcap, off);
   float4* bis = (float4*)(smem + off[0]);
-  float* bd2 = (float*)(smem + off[2]);
-  uint2* queue = (uint2*)(smem + off[5]);
+  unsigned* queue = (unsigned*)(smem + off[5]);
   unsigned* hist = (unsigned*)(smem + off[6]);
   Cell<PlaneStoreShared, TriCached> c;
   c.ps.bind((float4*)(smem + off[3]), lane);
@@ -989,7 +988,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         const unsigned pair = live ? a.pair_ids[i] : 0u;
         const unsigned long long mask = __ballot(live);
         const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
-        if (live) queue[(qtail + (unsigned)below) % kQueue] = make_uint2(pair, (unsigned)feed_slot);
+        if (live) queue[(qtail + (unsigned)below) % kQueue] = pair;
         qtail += (unsigned)__popcll(mask);
         cursor += 64;
         continue;
@@ -1011,20 +1010,15 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
       const int32_t* nbr = a.site_knn + (size_t)s * a.K;
       __syncthreads();  // the slot's last readers are done
       float4* eb = bis + other * K4;
-      float* db = bd2 + other * K4;
       for (int v = lane; v < K; v += 64) {
         const int z = nbr[v];
         if ((unsigned)z >= (unsigned)a.n_sites) {  // clipping stops at such a neighbour
-          eb[v] = make_float4(0.f, 0.f, 0.f, 0.f);
-          db[v] = kInvalidNbr;
+          const float nan = __int_as_float(0x7fc00000);
+          eb[v] = make_float4(nan, nan, nan, nan);
           continue;
         }
-        const P4 zp = site_at(a, z);
-        const P4 eq = bisector(sp, zp);
-        const P4 d = sub3(zp, sp);
-        const float dd = dot3(d, d);
+        const P4 eq = bisector(sp, site_at(a, z));
         eb[v] = make_float4(eq.x, eq.y, eq.z, eq.w);
-        db[v] = dd;
       }
       __syncthreads();
       if (other) site1 = s; else site0 = s;
@@ -1040,9 +1034,10 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
     const unsigned avail = qtail - qhead;
     const unsigned rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0));
     if (lst == kIdle && rank < avail) {
-      const uint2 q = queue[(qhead + rank) % kQueue];
-      ppair = q.x;
-      pslot = (int)q.y;
+      // the entries before qsplit were queued for the other slot's site
+      const unsigned qp = qhead + rank;
+      ppair = queue[qp % kQueue];
+      pslot = qp < qsplit ? feed_slot ^ 1 : feed_slot;
       pv = 0;
       lst = kClipping;
       ptet = ppair / (unsigned)a.TK;
@@ -1132,13 +1127,16 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         out = 1;
       } else {
         for (int it = 0; it < kTestsPerTrip && out == 0; ++it) {
-          const float d2z = bd2[pslot * K4 + pv];
-          if (d2z == kInvalidNbr) {
+          // |z - s|^2 from the bisector's normal s - z, the same bits as from
+          // z - s; NaN marks an out-of-range neighbour
+          const P4 e = ld4(c.ps.bis, pv);
+          const float d2z = dot3(e, e);
+          if (d2z != d2z) {
             c.status = kInconsistentBoundary;
             out = 2;
             break;
           }
-          const int r = c.clip_test(ld4(c.ps.bis, pv), d2z, conf, ksec);
+          const int r = c.clip_test(e, d2z, conf, ksec);
           if (r == 2) {
             cut = true;
             cut_d2z = d2z;
