@@ -1004,6 +1004,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         no_more_sites = true;
         break;
       }
+      if (a.site_order) s = a.site_order[s];
       const unsigned pb = a.pair_start[s], pe = a.pair_count ? pb + a.pair_count[s] : a.pair_start[s + 1];
       if (pb == pe) continue;
       const P4 sp = site_at(a, s);
@@ -1231,6 +1232,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
 // number.  Small and without polytope state, it runs at full occupancy, where
 // inside the clipping kernel the same test ran at under two waves per SIMD.
 constexpr int kCullMaxDepth = 256;
+constexpr int kXcds = 8;  // MI355X: accelerator complex dies, one L2 each
 
 __global__ __launch_bounds__(64) void k_cull_site(ClipArgs a, int depth, unsigned* pair_ids, unsigned* pair_count) {
   extern __shared__ __align__(16) uint8_t smem[];
@@ -1238,7 +1240,14 @@ __global__ __launch_bounds__(64) void k_cull_site(ClipArgs a, int depth, unsigne
   float4* ec = eb + depth;      // [depth] |z - s|^2, |n|_1, |w|, kCullMargin * |n|
   const int lane = threadIdx.x;
   unsigned n_culled = 0;  // over the block's sites: one histogram atomic per block
-  for (int s = blockIdx.x; s < a.n_sites; s += gridDim.x) {
+  // Blocks go to the 8 XCDs in turn, each with an L2 of its own: the blocks of
+  // one XCD take one contiguous stretch of the site order, so that the tets
+  // that nearby sites share are fetched into the same L2.
+  const int xcd = blockIdx.x % kXcds, per = gridDim.x / kXcds;
+  const int span = (a.n_sites + kXcds - 1) / kXcds;
+  const int s_begin = min(a.n_sites, xcd * span), s_end = min(a.n_sites, s_begin + span);
+  for (int si = s_begin + (int)blockIdx.x / kXcds; si < s_end; si += per) {
+    const int s = a.site_order ? a.site_order[si] : si;
     const unsigned pb = a.pair_start[s], pe = a.pair_start[s + 1];
     if (pb == pe) {
       if (lane == 0) pair_count[s] = 0;
@@ -1943,7 +1952,8 @@ hipError_t cull_sites(const ClipArgs& a, unsigned* pair_ids, unsigned* pair_coun
   const int K = a.site_depth > 0 ? std::min(a.site_depth, a.K) : a.K;
   const int depth = std::min(a.cull_depth, K);
   if (depth <= 0 || depth > kCullMaxDepth) return hipErrorInvalidValue;
-  const unsigned blocks = (unsigned)std::min(a.n_sites, 16384);
+  // a multiple of the XCD count, so that every XCD gets the same number of blocks
+  const unsigned blocks = (unsigned)(kXcds * std::max(1, std::min((a.n_sites + kXcds - 1) / kXcds, 16384 / kXcds)));
   hipLaunchKernelGGL(k_cull_site, dim3(blocks), dim3(64), 32 * (size_t)depth, st, a, depth, pair_ids, pair_count);
   return hipGetLastError();
 }

@@ -32,6 +32,9 @@ struct ClipArgs {
                              // test (0 = off)
   int filter;                // arithmetic filter of the conflict test (0 off, 1 reference, 2 FP64 re-check)
   unsigned* site_counter;    // site-major form: next site to take, zero before the launch
+  const int* site_order;     // site-major form and cull_sites, optional: the order in which sites are
+                             // taken (a permutation; neighbouring sites next to each other share
+                             // tets, which then stay in cache)
   int site_depth;            // site-major form: bisectors per site table (0 = K); a pair
                              // that needs more goes to retry_list
   // site-major form, optional: polytopes whose clipping ended are handed to

@@ -651,6 +651,8 @@ int vd_clip_step(vd_ctx* ctx) {
     ctx->ev_pairs = true;
     a.pair_start = ctx->p_start.p;
     a.pair_ids = ctx->p_ids.p;
+    // sites in the kNN grid's cell order: nearby sites share most of their tets
+    a.site_order = ctx->g_sorted_ids.p;
     if (c.form == VD_FORM_CELLS) {
       // kernel B (every site's cell), then kernel C (cell x tet per pair)
       VD_HIP(ctx->c_status.ensure(n), "alloc cells");
