@@ -1721,8 +1721,14 @@ __global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs) {
 // the global counter once; the scatter pass reserves one range per (block,
 // site) the same way.  Sites that do not fit the table go through global
 // atomics directly.  The order inside a site's list is unspecified.
-constexpr int kGroupPairs = 8192;
-constexpr int kHashBits = 12;
+#ifndef VD_GROUP_PAIRS
+#define VD_GROUP_PAIRS 8192
+#endif
+#ifndef VD_GROUP_HASH_BITS
+#define VD_GROUP_HASH_BITS 11
+#endif
+constexpr int kGroupPairs = VD_GROUP_PAIRS;
+constexpr int kHashBits = VD_GROUP_HASH_BITS;
 constexpr int kHashSlots = 1 << kHashBits;
 constexpr int kEmptySlot = -1;
 

@@ -316,7 +316,10 @@ constexpr int kQueryWaves = 4;
 // place of one that it excludes, mode 0 and 3), so its k-th key is below that
 // bound (rounded up by a 1e-5 margin) and candidates beyond it are never
 // staged, and the cell walk stops there.
-constexpr int kQueryRun = 8;
+#ifndef VD_KNN_RUN
+#define VD_KNN_RUN 8
+#endif
+constexpr int kQueryRun = VD_KNN_RUN;
 
 // After the first query of a run, the points that the run's other queries can
 // need are gathered once into LDS (at most kUnionCap): query i's k-th distance
