@@ -68,6 +68,10 @@ int tet_k_heuristic(int n_site, int n_tet, int preferred) {
 struct vd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream of the kNN step (the site queries beside the tet queries),
+  // forked from and joined back into `stream` through the two events
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   vd_config cfg{};
   std::string err;
 
@@ -334,6 +338,13 @@ int vd_ctx_create(vd_ctx** out, int device, const vd_config* cfg) {
     delete ctx;
     return VD_ERR_HIP;
   }
+  if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess) {
+    fprintf(stderr, "vd_ctx_create: cannot create the kNN step's second stream on device %d\n", device);
+    vd_ctx_destroy(ctx);
+    return VD_ERR_HIP;
+  }
   for (auto& row : ctx->ev)
     for (auto& e : row) (void)hipEventCreate(&e);
   if (ensure_offsets(ctx) != VD_OK || ctx->hist.ensure(8) != hipSuccess) {
@@ -350,6 +361,7 @@ void vd_ctx_destroy(vd_ctx* ctx) {
   ctx->drop_graph();
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->side) (void)hipStreamSynchronize(ctx->side);
   ctx->verts.release(); ctx->sites.release(); ctx->tet_xyz.release();
   ctx->b_site_begin.release(); ctx->b_voff.release(); ctx->b_cell_base.release(); ctx->b_site_inst.release(); ctx->b_tet_inst.release();
   ctx->b_box.release(); ctx->tets.release(); ctx->offsets.release();
@@ -369,6 +381,9 @@ void vd_ctx_destroy(vd_ctx* ctx) {
   for (auto& row : ctx->ev)
     for (auto& e : row)
       if (e) (void)hipEventDestroy(e);
+  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+  if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -581,14 +596,26 @@ int vd_knn_step(vd_ctx* ctx) {
   VD_HIP(vd::grid_build(gb, ctx->sites.p, n, ctx->n_verts ? ctx->mesh_lo : nullptr,
                         ctx->n_verts ? ctx->mesh_hi : nullptr, ctx->cfg.grid_density, ctx->stream),
          "grid build");
+  // The site and the tet queries read the same grid and write tables of their
+  // own: with both, the site queries run on the second stream beside the tet
+  // queries, so that neither launch's last partial wave of workgroups leaves
+  // the GPU idle, and the step ends when both have.
+  const bool tets = ctx->n_tets > 0 && ctx->TK > 0;
+  const bool fork = ctx->K > 0 && tets;
+  if (fork) {
+    VD_HIP(hipEventRecord(ctx->fork_ev, ctx->stream), "event");
+    VD_HIP(hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0), "stream wait");
+  }
   if (ctx->K > 0)
     VD_HIP(vd::knn_query(gb, n, ctx->offsets.p, ctx->n_offsets, ctx->K, 0, n, nullptr, nullptr, nullptr, nullptr,
-                         ctx->site_knn.p, 0, ctx->stream),
+                         ctx->site_knn.p, 0, fork ? ctx->side : ctx->stream),
            "site knn");
-  if (ctx->n_tets > 0 && ctx->TK > 0)
+  if (fork) VD_HIP(hipEventRecord(ctx->join_ev, ctx->side), "event");
+  if (tets)
     VD_HIP(vd::knn_query(gb, n, ctx->offsets.p, ctx->n_offsets, ctx->TK, 1, ctx->n_tets, ctx->verts.p, ctx->tets.p,
                          nullptr, ctx->batch() ? ctx->b_tet_inst.p : nullptr, ctx->tet_knn.p, 0, ctx->stream),
            "tet knn");
+  if (fork) VD_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0), "stream wait");
   VD_HIP(record_phase(ctx, (1)), "event");
   ctx->ev_knn = true;
   ctx->ev_pairs = ctx->ev_clip = ctx->ev_update = false;
