@@ -1748,17 +1748,6 @@ __device__ int hash_insert(int* keys, int site) {
   return -1;
 }
 
-__device__ int hash_find(const int* keys, int site) {
-  unsigned h = hash_slot(site);
-  for (int probe = 0; probe < kHashSlots; ++probe) {
-    const int k = keys[h];
-    if (k == site) return (int)h;
-    if (k == kEmptySlot) return -1;
-    h = (h + 1) & (kHashSlots - 1);
-  }
-  return -1;
-}
-
 __global__ __launch_bounds__(256) void k_pairs_count(const int32_t* tet_knn, long long n_pairs, int n_sites,
                                                       int* count) {
   __shared__ int keys[kHashSlots];
@@ -1785,42 +1774,98 @@ __global__ __launch_bounds__(256) void k_pairs_count(const int32_t* tet_knn, lon
     if (keys[i] != kEmptySlot) atomicAdd(count + keys[i], vals[i]);
 }
 
-__global__ __launch_bounds__(256) void k_pairs_scatter(const int32_t* tet_knn, long long n_pairs, int n_sites,
-                                                        const unsigned* start, int* cursor, unsigned* pair_ids) {
-  __shared__ int keys[kHashSlots];
-  __shared__ int vals[kHashSlots];
-  __shared__ unsigned offs[kHashSlots];
-  for (int i = threadIdx.x; i < kHashSlots; i += blockDim.x) {
+// The scatter pass takes kScatterPairs pairs per block: it ranks each pair
+// within its site's slot of the block's hash table, reserves one global range
+// per slot, places the pair ids in LDS in slot order and writes them out from
+// there, so that consecutive lanes write consecutive words of a site's range
+// (a store of one tet's 64 candidates would touch 64 cache lines).
+#ifndef VD_SCATTER_PAIRS
+#define VD_SCATTER_PAIRS 4096
+#endif
+constexpr int kScatterPairs = VD_SCATTER_PAIRS;
+constexpr int kScatterThreads = 256;
+constexpr int kScatterPerThread = kScatterPairs / kScatterThreads;
+static_assert(kScatterPairs % kScatterThreads == 0 && kScatterPairs <= 65536, "scatter group");
+static_assert(kHashSlots % kScatterThreads == 0 && kHashSlots <= 65536, "scatter hash table");
+
+__global__ __launch_bounds__(kScatterThreads) void k_pairs_scatter(const int32_t* tet_knn, long long n_pairs,
+                                                                   int n_sites, const unsigned* start, int* cursor,
+                                                                   unsigned* pair_ids) {
+  constexpr int kSlotsPerThread = kHashSlots / kScatterThreads;
+  __shared__ int keys[kHashSlots];         // site of the slot; after the reservation, its global
+                                           // offset minus its local base (mod 2^32)
+  __shared__ unsigned cnt[kHashSlots];     // pairs of the slot; after the scan, its local base
+  __shared__ unsigned stage[kScatterPairs];       // pair ids in slot order
+  __shared__ uint16_t stage_slot[kScatterPairs];  // their slots
+  unsigned* wsum = stage;  // the waves' count sums while the slots are scanned
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int i = tid; i < kHashSlots; i += kScatterThreads) {
     keys[i] = kEmptySlot;
-    vals[i] = 0;
+    cnt[i] = 0;
   }
   __syncthreads();
-  const long long base = (long long)blockIdx.x * kGroupPairs;
-  for (int j = threadIdx.x; j < kGroupPairs; j += blockDim.x) {
-    const long long p = base + j;
-    if (p >= n_pairs) break;
+  const long long base = (long long)blockIdx.x * kScatterPairs;
+  // slot (high 16 bits) and rank within it (low 16) of this thread's pairs
+  uint32_t code[kScatterPerThread];
+#pragma unroll
+  for (int k = 0; k < kScatterPerThread; ++k) {
+    const long long p = base + k * kScatterThreads + tid;
+    code[k] = ~0u;
+    if (p >= n_pairs) continue;
     const int s = tet_knn[p];
     if ((unsigned)s >= (unsigned)n_sites) continue;
     const int h = hash_insert(keys, s);
-    if (h >= 0) atomicAdd(&vals[h], 1);
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < kHashSlots; i += blockDim.x) {
-    if (keys[i] != kEmptySlot) {
-      offs[i] = start[keys[i]] + (unsigned)atomicAdd(cursor + keys[i], vals[i]);
-      vals[i] = 0;
+    if (h >= 0) {
+      code[k] = ((uint32_t)h << 16) | (uint32_t)atomicAdd(&cnt[h], 1u);
+    } else {
+      pair_ids[start[s] + (unsigned)atomicAdd(cursor + s, 1)] = (unsigned)p;
     }
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < kGroupPairs; j += blockDim.x) {
-    const long long p = base + j;
-    if (p >= n_pairs) break;
-    const int s = tet_knn[p];
-    if ((unsigned)s >= (unsigned)n_sites) continue;
-    const int h = hash_find(keys, s);
-    const unsigned pos = h >= 0 ? offs[h] + (unsigned)atomicAdd(&vals[h], 1) : start[s] + (unsigned)atomicAdd(cursor + s, 1);
-    pair_ids[pos] = (unsigned)p;
+  // exclusive scan of the slot counts (kSlotsPerThread consecutive slots per
+  // thread), then one global reservation per occupied slot
+  unsigned c[kSlotsPerThread], sum = 0;
+#pragma unroll
+  for (int k = 0; k < kSlotsPerThread; ++k) {
+    c[k] = cnt[tid * kSlotsPerThread + k];
+    sum += c[k];
   }
+  unsigned incl = sum;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned v = __shfl_up(incl, d);
+    if (lane >= d) incl += v;
+  }
+  if (lane == 63) wsum[wv] = incl;
+  __syncthreads();
+  unsigned run = incl - sum, total = 0;
+#pragma unroll
+  for (int w = 0; w < kScatterThreads / 64; ++w) {
+    if (w < wv) run += wsum[w];
+    total += wsum[w];
+  }
+#pragma unroll
+  for (int k = 0; k < kSlotsPerThread; ++k) {
+    const int j = tid * kSlotsPerThread + k;
+    const int key = keys[j];
+    cnt[j] = run;
+    if (key != kEmptySlot && c[k] != 0u) {
+      const unsigned off = start[key] + (unsigned)atomicAdd(cursor + key, (int)c[k]);
+      keys[j] = (int)(off - run);
+    }
+    run += c[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kScatterPerThread; ++k) {
+    if (code[k] == ~0u) continue;
+    const unsigned h = code[k] >> 16;
+    const unsigned j = cnt[h] + (code[k] & 0xffffu);
+    stage[j] = (unsigned)(base + k * kScatterThreads + tid);
+    stage_slot[j] = (uint16_t)h;
+  }
+  __syncthreads();
+  for (unsigned j = tid; j < total; j += kScatterThreads) pair_ids[(unsigned)keys[stage_slot[j]] + j] = stage[j];
 }
 
 __global__ __launch_bounds__(256) void k_srnr_rows(const unsigned* list, const unsigned* n_list, long long cap,
@@ -2004,8 +2049,9 @@ hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites,
   hipLaunchKernelGGL(k_pairs_count, dim3(blocks), dim3(256), 0, st, tet_knn, n_pairs, n_sites, count);
   hipError_t e = scan_excl(count, nullptr, n_sites, start, cursor, scan_tmp, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_pairs_scatter, dim3(blocks), dim3(256), 0, st, tet_knn, n_pairs, n_sites, start, cursor,
-                     pair_ids);
+  const unsigned sblocks = (unsigned)((n_pairs + kScatterPairs - 1) / kScatterPairs);
+  hipLaunchKernelGGL(k_pairs_scatter, dim3(sblocks), dim3(kScatterThreads), 0, st, tet_knn, n_pairs, n_sites, start,
+                     cursor, pair_ids);
   return hipGetLastError();
 }
 
