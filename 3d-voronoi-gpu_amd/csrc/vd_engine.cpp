@@ -65,6 +65,23 @@ int tet_k_heuristic(int n_site, int n_tet, int preferred) {
 
 }  // namespace
 
+namespace vdi {
+
+// By default the largest vertex cap (at most 32 and the configured cap) whose
+// LDS at the configured table depth, in the 512-byte granules a CU allocates,
+// still lets 9 waves share a CU's 160 KB: 17 at 40 bisectors.  A batch whose
+// instances have fewer neighbours has smaller tables and the same cap.
+constexpr int kSiteWavesPerCu = 9;
+int site_tcap_of(const vd_config& c) {
+  if (c.site_tcap > 0) return std::min(c.site_tcap, c.tcap);
+  const int depth = std::max(site_depth_of(c, c.site_k), 1);
+  int t = std::min(c.tcap, 32);
+  while (t > 8 && ((vd::clip_site_lds(depth, t) + 511) & ~(size_t)511) * kSiteWavesPerCu > 160 * 1024) --t;
+  return t;
+}
+
+}  // namespace vdi
+
 struct vd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -200,7 +217,7 @@ int check_config(vd_ctx* ctx, const vd_config& c) {
   // the tet-major kernel also serves the site-major form's overflow retries
   if (vd::clip_plain_lds(c.pcap, c.tcap) > 160 * 1024)
     return ctx->fail(VD_ERR_ARG, "pcap/tcap too large for the tet-major kernel's LDS");
-  const int fast_tcap = c.site_tcap > 0 ? std::min(c.site_tcap, c.tcap) : std::min(c.tcap, 16);
+  const int fast_tcap = vdi::site_tcap_of(c);
   if (c.form == VD_FORM_SITE_MAJOR && vd::clip_site_lds(site_depth_of(c, c.site_k), fast_tcap) > 160 * 1024)
     return ctx->fail(VD_ERR_ARG, "site_k/site_tcap too large for the site-major form's LDS");
   if (c.cull_depth < 0 || c.cull_depth > 256)
@@ -701,7 +718,7 @@ int vd_clip_step(vd_ctx* ctx) {
     VD_HIP(ctx->site_counter.ensure(1), "alloc site counter");
     VD_HIP(vd::fill_u32(ctx->site_counter.p, 1, 0u, ctx->stream), "clear site counter");
     a.site_counter = ctx->site_counter.p;
-    const int fast_tcap = c.site_tcap > 0 ? std::min(c.site_tcap, c.tcap) : std::min(c.tcap, 16);
+    const int fast_tcap = vdi::site_tcap_of(c);
     a.site_depth = site_depth_of(c, ctx->K);
     if (fast_tcap < c.tcap || a.site_depth < ctx->K) {
       VD_HIP(ctx->retry_list.ensure(std::max(n_pairs, 1LL)), "alloc retry list");

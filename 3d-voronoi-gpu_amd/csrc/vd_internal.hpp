@@ -30,5 +30,8 @@ int assemble(vd_ctx* ctx, const JobDev* jobs, int n);
 int scatter_sites(vd_ctx* ctx, const JobDev* jobs, const int* idx, int n_idx);
 hipStream_t stream(vd_ctx* ctx);
 void set_error(vd_ctx* ctx, const char* msg);
+// Vertex cap of the site-major launch under configuration c (its site_tcap,
+// or the automatic choice).
+int site_tcap_of(const vd_config& c);
 
 }  // namespace vdi

@@ -83,7 +83,7 @@ struct vd_server {
     const size_t own = 16 * (size_t)nv + (16 + 48) * (size_t)nt + 16 * (size_t)ns + 4 * 256;
     size_t rec = 0;
     if (cfg.form == VD_FORM_SITE_MAJOR && cfg.site_records >= 0) {
-      const int ft = cfg.site_tcap > 0 ? std::min(cfg.site_tcap, cfg.tcap) : std::min(cfg.tcap, 16);
+      const int ft = vdi::site_tcap_of(cfg);
       rec = (16 + 4 * (size_t)vd::rec_stride(ft)) * TK / 10;
     }
     const size_t work =

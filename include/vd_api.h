@@ -71,7 +71,9 @@ typedef struct vd_config {
   int clip_blocks;    /* workgroups of the site-major launch; 0 = automatic */
   int site_tcap;      /* vertex cap of the site-major launch, at most 32; pairs that
                          exceed it are recomputed at tcap, so results do not depend
-                         on it; 0 = automatic (min(tcap, 16)) */
+                         on it; 0 = automatic: the largest (at most 32) whose LDS at
+                         the configured table depth lets 9 waves share a CU, 17 at
+                         site_depth 40 */
   int cull_depth;     /* site-major form: a pair whose tet lies entirely beyond one of the
                          seed's first cull_depth bisectors (by more than the FP32 rounding
                          bound plus 0.01) has an empty intersection and is not clipped;

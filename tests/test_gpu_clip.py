@@ -59,6 +59,27 @@ def test_pairs_bit_exact(vdn, O, case, form, accum, moments):
         np.testing.assert_allclose(g["sites"], O.lloyd_update(r["site_vals"], sites, fp32=True), rtol=1e-5, atol=1e-4)
 
 
+@pytest.mark.parametrize("form", ["site", "cells"])
+def test_grouping_table_overflow(vdn, O, form):
+    """Tets in random order with 8 candidates each: a grouping block's pairs
+    then name more distinct sites than its LDS hash table holds, so part of
+    them go through the global counters directly; every pair must still be
+    grouped exactly once (per-pair results equal the oracle's)."""
+    verts, tets, sites = problem("cube", 20, 20000)
+    tets = tets[np.random.default_rng(3).permutation(len(tets))]
+    sk = O.site_knn(sites, 90)
+    tk = O.tet_knn(verts, tets, sites, 8)
+    print(f"form={form} tets={len(tets)} pairs={tk.size}", flush=True)
+    g = _run(vdn, verts, tets, sites, tet_k=8, form=form, accum="f64")
+    np.testing.assert_array_equal(g["tknn"], tk)
+    # each form against the oracle's restatement of the same formulation
+    clip = O.clip_tets if form == "site" else O.clip_cells
+    r = clip(sites, sk, verts, tets, tk, geom="f32", pair_acc="f64", site_mode=O.SITE_FIXED)
+    np.testing.assert_array_equal(g["status"].astype(np.int32), r["pair_status"])
+    np.testing.assert_array_equal(g["vals"], r["pair_vals"])
+    np.testing.assert_array_equal(g["moments"], r["site_vals"])
+
+
 @pytest.mark.parametrize("caps", [(8, 12), (10, 16)])
 def test_overflow_statuses_match(vdn, O, caps):
     """Tiny caps force vertex/triangle overflows; statuses must agree.

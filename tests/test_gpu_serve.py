@@ -47,10 +47,11 @@ def test_jobs_in_the_mix_equal_jobs_alone(vdn):
 
 def _footprint(v, t, st, K=90, TK=32):
     """The server's estimate of a running job's device bytes (site-major
-    form, volume-pass records of 80 bytes for one pair in ten)."""
+    form, volume-pass records of 16 + 4 * 20 bytes, at the automatic vertex
+    cap of 17, for one pair in ten)."""
     nv, nt, ns = len(v), len(t), len(st)
     own = 16 * nv + 64 * nt + 16 * ns + 1024
-    work = 16 * nv + (16 + 48 + 12 * TK + 8 * TK + 8) * nt + (16 + 4 * K + 64) * ns
+    work = 16 * nv + (16 + 48 + 12 * TK + (16 + 4 * 20) * TK // 10 + 8) * nt + (16 + 4 * K + 64) * ns
     return own + work
 
 
