@@ -1951,7 +1951,8 @@ static hipError_t launch_site(Kern k, int n_blocks, size_t lds, hipStream_t st, 
   return hipGetLastError();
 }
 
-hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, int n_blocks, hipStream_t st) {
+hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, int n_blocks, hipStream_t st,
+                       bool volume_pass) {
   if (a.n_pairs <= 0) return hipSuccess;
   if (form == 0) {
     const size_t lds = clip_plain_lds(a.pcap, a.tcap);
@@ -1983,8 +1984,13 @@ hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, i
     e = launch_site(k_clip_site<1, 0>, n_blocks, lds, st, a);
   else
     e = launch_site(k_clip_site<1, 1>, n_blocks, lds, st, a);
-  if (e != hipSuccess || !a.rec_meta || a.rec_cap == 0) return e;
-  // the records' volume pass; the record count is read on the device
+  if (e != hipSuccess || !volume_pass) return e;
+  return site_volume_pass(a, moments, acc_mode, st);
+}
+
+hipError_t site_volume_pass(const ClipArgs& a, int moments, int acc_mode, hipStream_t st) {
+  if (a.n_pairs <= 0 || !a.rec_meta || a.rec_cap == 0) return hipSuccess;
+  // the record count is read on the device
   const dim3 grid((unsigned)std::min<long long>(((long long)a.rec_cap + kMomentsBlock - 1) / kMomentsBlock, 4096)),
       block(kMomentsBlock);
   if (moments == 0 && acc_mode == 0)

@@ -90,8 +90,12 @@ __host__ __device__ inline int rec_stride(int tcap) { return (tcap + 3) & ~3; }
 // acc_mode: 0 FP32 atomics, 1 FP64 pair sums + fixed-point site sums.
 // n_blocks: workgroups of the site-major form (0 = as many as stay resident),
 // and of the tet-major form when it runs over a.pair_list.  The site-major
-// form with records (a.rec_meta) also launches their volume pass.
-hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, int n_blocks, hipStream_t st);
+// form with records (a.rec_meta) also launches their volume pass, unless
+// volume_pass is false (the caller then launches site_volume_pass).
+hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, int n_blocks, hipStream_t st,
+                       bool volume_pass = true);
+// The volume pass over the records of a site-major launch with arguments a.
+hipError_t site_volume_pass(const ClipArgs& a, int moments, int acc_mode, hipStream_t st);
 
 // Groups the pairs of an n_tets x TK candidate table (at most 2^32 - 1 pairs)
 // by site: site s owns pair_ids[start[s] .. start[s+1]).  count must be zero; count/cursor hold

@@ -769,7 +769,20 @@ int vd_clip_step(vd_ctx* ctx) {
     a.srnr_list = ctx->srnr_list.p;
     a.srnr_count = ctx->srnr_count.p;
   }
-  VD_HIP(vd::clip_launch(a, c.form, c.moments, c.accum, blocks, ctx->stream), "clip");
+  // The site-major form's re-clip of its overflow pairs depends on the
+  // clipping launch only: it runs on the second stream beside the volume pass
+  // over the clipping's records (both only add to the site sums), and the
+  // step waits for both.
+  const bool vol_pass = c.form == VD_FORM_SITE_MAJOR && a.rec_meta && a.rec_cap > 0;
+  const bool beside = vol_pass && a.retry_list;
+  VD_HIP(vd::clip_launch(a, c.form, c.moments, c.accum, blocks, ctx->stream, !beside), "clip");
+  hipStream_t rs = ctx->stream;
+  if (beside) {
+    VD_HIP(hipEventRecord(ctx->fork_ev, ctx->stream), "event");
+    VD_HIP(hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0), "stream wait");
+    rs = ctx->side;
+    VD_HIP(vd::site_volume_pass(a, c.moments, c.accum, ctx->stream), "volume pass");
+  }
   VD_HIP(record_phase(ctx, (5)), "event");
   if (a.retry_list) {
     // pairs that overflowed the site-major launch's vertex cap or needed more
@@ -791,11 +804,11 @@ int vd_clip_step(vd_ctx* ctx) {
       if (mid < c.tcap || pmid < c.pcap) {
         VD_HIP(ctx->retry_list2.ensure(std::max(n_pairs, 1LL)), "alloc retry list");
         VD_HIP(ctx->retry_count2.ensure(1), "alloc retry list");
-        VD_HIP(vd::fill_u32(ctx->retry_count2.p, 1, 0u, ctx->stream), "clear retry count");
+        VD_HIP(vd::fill_u32(ctx->retry_count2.p, 1, 0u, rs), "clear retry count");
         r.retry_list = ctx->retry_list2.p;
         r.retry_count = ctx->retry_count2.p;
       }
-      VD_HIP(vd::clip_launch(r, vd::kFormTetCached, c.moments, c.accum, 768, ctx->stream), "clip retry");
+      VD_HIP(vd::clip_launch(r, vd::kFormTetCached, c.moments, c.accum, 768, rs), "clip retry");
       if (mid < c.tcap || pmid < c.pcap) {
         vd::ClipArgs r3 = a;
         r3.tcap = c.tcap;
@@ -803,12 +816,16 @@ int vd_clip_step(vd_ctx* ctx) {
         r3.pair_list_count = ctx->retry_count2.p;
         r3.retry_list = nullptr;
         r3.retry_count = nullptr;
-        VD_HIP(vd::clip_launch(r3, VD_FORM_TET_MAJOR, c.moments, c.accum, 256, ctx->stream), "clip retry");
+        VD_HIP(vd::clip_launch(r3, VD_FORM_TET_MAJOR, c.moments, c.accum, 256, rs), "clip retry");
       }
     } else {
       r.tcap = c.tcap;
-      VD_HIP(vd::clip_launch(r, VD_FORM_TET_MAJOR, c.moments, c.accum, 256, ctx->stream), "clip retry");
+      VD_HIP(vd::clip_launch(r, VD_FORM_TET_MAJOR, c.moments, c.accum, 256, rs), "clip retry");
     }
+  }
+  if (beside) {
+    VD_HIP(hipEventRecord(ctx->join_ev, ctx->side), "event");
+    VD_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0), "stream wait");
   }
   if (K2 > 0) {
     VD_HIP(ctx->srnr_row_of.ensure(n), "alloc srnr rows");
