@@ -408,15 +408,15 @@ struct Cell {
       }
 #pragma unroll
       for (int j = 0; j < kFarUnroll; ++j) d[j] = ts.d2(t[j]);
+      // as selects rather than branches: per slot, found inside the near
+      // ball (cleared from far) or a candidate (possibly blocking)
 #pragma unroll
       for (int j = 0; j < kFarUnroll; ++j) {
-        if (!has[j]) continue;
-        if (d[j] < near2) {
-          far &= ~(1u << t[j]);
-        } else {
-          cand |= 1u << t[j];
-          if (4.f * d[j] >= d2z) block |= 1u << t[j];
-        }
+        const uint32_t bit = has[j] ? 1u << t[j] : 0u;
+        const uint32_t keep = d[j] < near2 ? 0u : bit;
+        far &= ~(bit & ~keep);
+        cand |= keep;
+        block |= 4.f * d[j] >= d2z ? keep : 0u;
       }
     }
     // the determinant only for candidates: a wave pays for the largest
@@ -435,9 +435,19 @@ struct Cell {
       }
 #pragma unroll
       for (int j = 0; j < kCandUnroll; ++j) tt[j] = tri(t[j]);
+      if (filter) {
 #pragma unroll
-      for (int j = 0; j < kCandUnroll; ++j)
-        if (has[j] && conflict(tt[j], eqn)) conf |= 1u << t[j];
+        for (int j = 0; j < kCandUnroll; ++j)
+          if (has[j] && conflict(tt[j], eqn)) conf |= 1u << t[j];
+      } else {
+        // without the filter the test has no side effect: every slot's
+        // determinant (slot 0 stands in for a missing one) and a select
+#pragma unroll
+        for (int j = 0; j < kCandUnroll; ++j) {
+          const float det = conflict_det(vertex_minors(ps.get(tri_a(tt[j])), ps.get(tri_b(tt[j])), ps.get(tri_c(tt[j]))), eqn);
+          conf |= has[j] && det > 0.f ? 1u << t[j] : 0u;
+        }
+      }
     }
     const bool kept_secure = d2z > 0.f && (block & ~conf) == 0u;
     if (status == kNeedsExactPredicates) {
