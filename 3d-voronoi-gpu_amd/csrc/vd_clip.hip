@@ -332,36 +332,25 @@ struct Cell {
       if (!op0 && !op1 && on1) simple = false;
       if (!op1 && !op2 && on2) simple = false;
       if (!op2 && !op0 && on0) simple = false;
-      if (!op0 && !op1 && !op2) {
-        if (first == kNoPlane) {
-          set_nxt(e0, e1);
-          set_nxt(e1, e2);
-          set_nxt(e2, e0);
-          first = e0;
-        } else {
-          simple = false;
-        }
-      }
+      const bool none = !op0 && !op1 && !op2;
+      if (none && first != kNoPlane) simple = false;
       if (!simple) {
         ++cur;
         if (cur == nt + nr) cur = nt;
         continue;
       }
-      if (!op0) set_nxt(e0, e1);
-      if (!op1) set_nxt(e1, e2);
-      if (!op2) set_nxt(e2, e0);
-      if (op0 && op1) {
-        if (first == e1) first = nxt(e1);
-        set_nxt(e1, kNoPlane);
-      }
-      if (op1 && op2) {
-        if (first == e2) first = nxt(e2);
-        set_nxt(e2, kNoPlane);
-      }
-      if (op2 && op0) {
-        if (first == e0) first = nxt(e0);
-        set_nxt(e0, kNoPlane);
-      }
+      // The merge as selects: a link e -> e' is added where the loop does
+      // not already run e' -> e, a plane between two such edges leaves the
+      // loop, and `first` steps past a plane that leaves (each plane's link
+      // is still its n value where it is read: its own edge is in the loop).
+      // The first triangle of an empty loop is the loop.
+      if (none) first = e0;
+      if (op0 && op1 && first == e1) first = n1;
+      if (op1 && op2 && first == e2) first = n2;
+      if (op2 && op0 && first == e0) first = n0;
+      set_nxt(e0, op2 && op0 ? kNoPlane : (op0 ? n0 : e1));
+      set_nxt(e1, op0 && op1 ? kNoPlane : (op1 ? n1 : e2));
+      set_nxt(e2, op1 && op2 ? kNoPlane : (op2 ? n2 : e0));
       ts.swap(cur, nt + nr - 1);
       cur = nt;
       --nr;
@@ -490,15 +479,12 @@ struct Cell {
       const int i = __builtin_ctz(conf);
       --nt;
       ++nr;
+      // slot i takes slot nt's triangle and flags (a no-op swap when i == nt)
       const uint32_t bi = 1u << i, bn = 1u << nt;
-      if (i != nt) {
-        ts.swap(i, nt);
-        conf = (conf & ~(bi | bn)) | ((conf & bn) ? bi : 0u);
-        far = (far & ~(bi | bn)) | ((far & bn) ? bi : 0u);
-      } else {
-        conf &= ~bi;
-        far &= ~bi;
-      }
+      const bool mv = i != nt;
+      ts.swap(i, nt);
+      conf = (conf & ~(bi | bn)) | (mv && (conf & bn) ? bi : 0u);
+      far = (far & ~(bi | bn)) | (mv && (far & bn) ? bi : 0u);
     }
     if (nt == 0) {
       status = kNoIntersection;
