@@ -500,29 +500,28 @@ struct Cell {
     // registers (c is the previous vertex's n), one plane read per vertex
     const P4 pp = ps.get(pid);
     P4 pc = ps.get(first);
+    // one exit for the three failures (an open or runaway loop, the vertex
+    // cap), taken before anything is stored for the failing vertex
+    // (err: the failure status, or -1; kTriangleOverflow is 0)
+    int err = -1;
     do {
-      if (c == kNoPlane || guard++ > pcap) {
-        status = kInconsistentBoundary;
-        return -1;
-      }
-      if (nt + 1 >= tcap) {
-        status = kTriangleOverflow;
-        return -1;
-      }
-      const int n = nxt(c);
-      if (n == kNoPlane) {
-        status = kInconsistentBoundary;
-        return -1;
-      }
+      err = (c == kNoPlane || guard++ > pcap) ? kInconsistentBoundary : (nt + 1 >= tcap ? kTriangleOverflow : -1);
+      const int n = err >= 0 ? kNoPlane : nxt(c);
+      if (err < 0 && n == kNoPlane) err = kInconsistentBoundary;
+      if (err >= 0) break;
       const P4 pn = ps.get(n);
       const float d = seed_d2(vertex3(pp, pc, pn));
       ts.put(nt, pack3(pid, c, n), d);
-      if (!(d < near2)) far |= 1u << nt;
-      new_block = new_block || (4.f * d >= d2z);
+      far |= d < near2 ? 0u : 1u << nt;
+      new_block |= 4.f * d >= d2z;
       ++nt;
       c = n;
       pc = pn;
     } while (c != first);
+    if (err >= 0) {
+      status = err;
+      return -1;
+    }
     return (kept_secure && !new_block) ? 1 : 0;
   }
 
@@ -1304,26 +1303,23 @@ __global__ __launch_bounds__(64) void k_cull_site(ClipArgs a, int depth, unsigne
         // |n|_1 * max|x| + |w|: three roundings of partial sums bounded by
         // that) plus the required depth beyond the plane (kCullMargin in
         // distance, times |n|)
+        // (as selects: a pair's loads are in flight together and every lane
+        // runs the same instructions)
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-          if (!open[h]) continue;
-          if (f.x == kInvalidNbr || 4.f * vmin2[h] <= f.x) {
-            // an invalid neighbour ends the clipping there; and once the
-            // tet's nearest corner is within |z - s| / 2 of the seed it is on
-            // the seed's side of this bisector and of every later one (the
-            // neighbour distances do not decrease)
-            open[h] = false;
-            continue;
-          }
+          // an invalid neighbour ends the clipping there; and once the tet's
+          // nearest corner is within |z - s| / 2 of the seed it is on the
+          // seed's side of this bisector and of every later one (the
+          // neighbour distances do not decrease)
+          const bool stop = f.x == kInvalidNbr || 4.f * vmin2[h] <= f.x;
           const float lim = -fmaf(kCullRound, fmaf(f.y, pmax[h], f.z), f.w);
           float hi = -INFINITY;
 #pragma unroll
           for (int k = 0; k < 4; ++k)
             hi = fmaxf(hi, fmaf(e.z, v[h][k].z, fmaf(e.y, v[h][k].y, fmaf(e.x, v[h][k].x, e.w))));
-          if (hi < lim) {
-            culled[h] = true;
-            open[h] = false;
-          }
+          const bool cut = open[h] && !stop && hi < lim;
+          culled[h] = culled[h] || cut;
+          open[h] = open[h] && !stop && !cut;
         }
       }
 #pragma unroll
