@@ -46,6 +46,30 @@ __device__ __forceinline__ uint64_t bperm64(uint64_t v, int src_lane) {
   const uint32_t hi = bperm32((uint32_t)(v >> 32), src_lane);
   return ((uint64_t)hi << 32) | lo;
 }
+
+// Value of lane (lane ^ j) for a compile-time j: within a row of 16 lanes
+// through DPP moves (quad permutes for 1 and 2; row rotations, where lane i
+// reads lane i - n of its row, for 4 and 8), which stay in the VALU, and
+// through the LDS crossbar (ds_bpermute) for 16 and 32.
+__device__ __forceinline__ uint32_t xor_lane32(uint32_t v, int j, int lane) {
+  const int x = (int)v;
+  switch (j) {
+    case 1: return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    case 2: return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    case 4: {
+      const int from_below = __builtin_amdgcn_mov_dpp(x, 0x124, 0xF, 0xF, false);  // row_ror:4, lane i - 4
+      const int from_above = __builtin_amdgcn_mov_dpp(x, 0x12C, 0xF, 0xF, false);  // row_ror:12, lane i + 4
+      return (uint32_t)((lane & 4) ? from_below : from_above);
+    }
+    case 8: return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);  // row_ror:8
+    default: return bperm32(v, lane ^ j);
+  }
+}
+__device__ __forceinline__ uint64_t xor_lane64(uint64_t v, int j, int lane) {
+  const uint32_t lo = xor_lane32((uint32_t)v, j, lane);
+  const uint32_t hi = xor_lane32((uint32_t)(v >> 32), j, lane);
+  return ((uint64_t)hi << 32) | lo;
+}
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -233,7 +257,7 @@ __device__ __forceinline__ uint64_t wave_sort64(uint64_t x, int lane) {
   for (int k = 2; k <= 64; k <<= 1) {
 #pragma unroll
     for (int j = k >> 1; j > 0; j >>= 1) {
-      const uint64_t o = bperm64(x, lane ^ j);
+      const uint64_t o = xor_lane64(x, j, lane);
       const bool up = (lane & k) == 0;
       const bool lower = (lane & j) == 0;
       x = (lower == up) ? umin64(x, o) : umax64(x, o);
@@ -270,7 +294,7 @@ __device__ __forceinline__ void topk_merge(TopK<M>& L, uint64_t x, int lane, boo
     const bool lower = (lane & j) == 0;
 #pragma unroll
     for (int s = 0; s < M; ++s) {
-      const uint64_t o = bperm64(L.s[s], lane ^ j);
+      const uint64_t o = xor_lane64(L.s[s], j, lane);
       L.s[s] = lower ? umin64(L.s[s], o) : umax64(L.s[s], o);
     }
   }
