@@ -246,7 +246,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+                                   "(sharding over several GPUs is not built)"}))
         sys.exit(2)
 
     from voronoi_amd import meshgen, native
