@@ -70,9 +70,51 @@ __device__ __forceinline__ uint64_t xor_lane64(uint64_t v, int j, int lane) {
   const uint32_t hi = xor_lane32((uint32_t)(v >> 32), j, lane);
   return ((uint64_t)hi << 32) | lo;
 }
+
+// Inclusive prefix sum across the wave through DPP moves: row shifts by 1, 2,
+// 4 and 8 within each row of 16 lanes (lanes without a source add 0), then
+// the last lane of row 0 into row 1 and of row 2 into row 3 (row_bcast:15),
+// then the last lane of row 1 into rows 2 and 3 (row_bcast:31).
+__device__ __forceinline__ int wave_incl_sum(int x) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);  // row_bcast:15, rows 1 and 3
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);  // row_bcast:31, rows 2 and 3
+  return x;
+}
+// Inclusive prefix maximum across the wave, the same DPP steps (lanes
+// without a source contribute -1).
+__device__ __forceinline__ int wave_incl_max(int x) {
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x111, 0xF, 0xF, false));
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x112, 0xF, 0xF, false));
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x114, 0xF, 0xF, false));
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x118, 0xF, 0xF, false));
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x142, 0xA, 0xF, false));
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x143, 0xC, 0xF, false));
+  return x;
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
+}
+
+// Owner lane of flat index f0 + lane of the cells' concatenated point ranges
+// (the largest lane whose range starts at or before it): every lane with
+// points whose range starts in [f0, f0 + 64) marks its start in `mark`, a
+// prefix maximum over the marks finds the latest start, and `carry` (the
+// owner of f0 - 1, 0 at first) covers ranges begun in earlier batches.
+__device__ __forceinline__ int batch_owner(int* mark, int lane, int cnt, int excl, int f0, int& carry) {
+  mark[lane] = -1;
+  wave_sync();
+  if (cnt > 0 && excl >= f0 && excl < f0 + 64) mark[excl - f0] = lane;
+  wave_sync();
+  const int own = max(wave_incl_max(mark[lane]), carry);
+  carry = __builtin_amdgcn_readlane(own, 63);
+  wave_sync();
+  return own;
 }
 __device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 __device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a < b ? b : a; }
@@ -390,10 +432,12 @@ __device__ __forceinline__ void query_point(const QueryArgs& a, int q, float4& q
 template <int M>
 __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
   __shared__ uint64_t stage_all[kQueryWaves][128];
+  __shared__ int mark_all[kQueryWaves][64];  // batch_owner's marks
   __shared__ float4 uni_all[kQueryWaves][kUnionCap];  // (x, y, z, id bits)
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   uint64_t* stage = stage_all[wv];
+  int* mark = mark_all[wv];
   float4* uni = uni_all[wv];
   const int q0 = (blockIdx.x * kQueryWaves + wv) * kQueryRun;
   int nq = a.n_query;
@@ -495,25 +539,13 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
       }
     }
     // exclusive prefix of the counts across the wave
-    int incl = cnt;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int v = __shfl_up(incl, d);
-      if (lane >= d) incl += v;
-    }
+    const int incl = wave_incl_sum(cnt);
     const int excl = incl - cnt;
     const int total = __shfl(incl, 63);
+    int carry = 0;
     for (int f0 = 0; f0 < total; f0 += 64) {
       const int f = f0 + lane;
-      // owner lane of flat index f: the largest lane whose prefix is <= f
-      // (every lane runs the search so that no shuffle reads an idle lane)
-      int own = 0;
-#pragma unroll
-      for (int step = 32; step > 0; step >>= 1) {
-        const int cand = own + step;
-        const int pe = __shfl(excl, cand & 63);
-        if (cand < 64 && pe <= f) own = cand;
-      }
+      const int own = batch_owner(mark, lane, cnt, excl, f0, carry);
       const int o_st = __shfl(st, own), o_ex = __shfl(excl, own);
       uint64_t key = ~0ull;
       bool ok = false;
@@ -621,23 +653,13 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
             cnt = a.start[cell + 1] - st;
           }
         }
-        int incl = cnt;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-          const int v = __shfl_up(incl, d);
-          if (lane >= d) incl += v;
-        }
+        const int incl = wave_incl_sum(cnt);
         const int excl = incl - cnt;
         const int total = __shfl(incl, 63);
+        int carry = 0;
         for (int f0 = 0; f0 < total && fits; f0 += 64) {
           const int f = f0 + lane;
-          int own = 0;
-#pragma unroll
-          for (int step = 32; step > 0; step >>= 1) {
-            const int cand = own + step;
-            const int pe = __shfl(excl, cand & 63);
-            if (cand < 64 && pe <= f) own = cand;
-          }
+          const int own = batch_owner(mark, lane, cnt, excl, f0, carry);
           const int o_st = __shfl(st, own), o_ex = __shfl(excl, own);
           bool keep = false;
           float4 p = {0.f, 0.f, 0.f, 0.f};
