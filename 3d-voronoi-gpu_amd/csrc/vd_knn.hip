@@ -383,7 +383,7 @@ constexpr int kQueryWaves = 4;
 // bound (rounded up by a 1e-5 margin) and candidates beyond it are never
 // staged, and the cell walk stops there.
 #ifndef VD_KNN_RUN
-#define VD_KNN_RUN 8
+#define VD_KNN_RUN 6
 #endif
 constexpr int kQueryRun = VD_KNN_RUN;
 
