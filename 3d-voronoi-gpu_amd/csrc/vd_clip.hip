@@ -2002,7 +2002,11 @@ hipError_t cull_sites(const ClipArgs& a, unsigned* pair_ids, unsigned* pair_coun
   const int depth = std::min(a.cull_depth, K);
   if (depth <= 0 || depth > kCullMaxDepth) return hipErrorInvalidValue;
   // a multiple of the XCD count, so that every XCD gets the same number of blocks
-  const unsigned blocks = (unsigned)(kXcds * std::max(1, std::min((a.n_sites + kXcds - 1) / kXcds, 16384 / kXcds)));
+#ifndef VD_CULL_BLOCKS
+#define VD_CULL_BLOCKS 32768
+#endif
+  const unsigned blocks =
+      (unsigned)(kXcds * std::max(1, std::min((a.n_sites + kXcds - 1) / kXcds, VD_CULL_BLOCKS / kXcds)));
   hipLaunchKernelGGL(k_cull_site, dim3(blocks), dim3(64), 32 * (size_t)depth, st, a, depth, pair_ids, pair_count);
   return hipGetLastError();
 }
