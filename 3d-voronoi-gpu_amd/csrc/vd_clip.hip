@@ -1147,7 +1147,14 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = ppair;
         lst = kIdle;
       } else if (out == 1 && c.status != kNoIntersection) {
-        defer = true;
+        // A polytope whose cut overflowed this launch's vertex cap may go to
+        // the retry list after its volume is known: that is decided here, in
+        // place, so that the list is final when this kernel ends (the re-clip
+        // that reads it runs beside the volume pass, not after it).
+        if (c.status == kTriangleOverflow && a.retry_list && a.tcap < a.tcap_full)
+          moments_now();
+        else
+          defer = true;
       } else if (out != 0) {
         PairSums<ACC> S;
         finish(S, false);
@@ -1459,7 +1466,8 @@ __global__ __launch_bounds__(kMomentsBlock) void k_site_moments(ClipArgs a) {
         status = kNoIntersection;
       else
         acc = true;
-      // then as the site-major kernel's finish step
+      // then as the site-major kernel's finish step (which keeps the records
+      // that could go to the retry list to itself)
       if (status == kTriangleOverflow && a.retry_list && a.tcap < a.tcap_full) {
         const unsigned slot = atomicAdd(a.retry_count, 1u);
         if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = pair;
