@@ -373,7 +373,10 @@ __device__ __forceinline__ float cell_bound(const GridShape& g, const float4& q,
   return s * (1.f - 1e-5f);
 }
 
-constexpr int kQueryWaves = 4;
+#ifndef VD_KNN_WAVES
+#define VD_KNN_WAVES 4
+#endif
+constexpr int kQueryWaves = VD_KNN_WAVES;
 
 // Consecutive queries (tets of one region, or sites of one grid cell) are
 // close to each other: a wave takes a run of kQueryRun of them, and each
