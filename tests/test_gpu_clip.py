@@ -120,6 +120,28 @@ def test_site_form_overflow_retry(vdn, O, site_tcap):
     assert [g["hist"][k] for k in vdn.STATUS_NAMES] == hist.tolist()
 
 
+def test_site_form_overflow_retry_repeated(vdn, O):
+    """The overflow re-clip runs on a second stream beside the volume pass over
+    a retry list that the clipping launch completes: repeated clip steps over
+    the same kNN give the oracle's per-pair results every time (a pair left
+    out of the re-clip would show as a status or value difference)."""
+    verts, tets, sites = problem("cube", 20, 2000)
+    sk, tk = oracle_knn("cube", 20, 2000, 90, 64)
+    r = O.clip_tets(sites, sk, verts, tets, tk, geom="f32", pair_acc="f64", site_mode=O.SITE_FIXED)
+    e = vdn.Engine(record_pairs=1, tet_k=64, form="site", accum="f64", site_tcap=8)
+    e.set_mesh(verts, tets)
+    e.set_sites(sites)
+    e.knn_step()
+    for rep in range(4):
+        print(f"clip step {rep}", flush=True)
+        e.clip_step()
+        st, vals = e.pairs()
+        np.testing.assert_array_equal(st.astype(np.int32), r["pair_status"])
+        np.testing.assert_array_equal(vals, r["pair_vals"])
+        np.testing.assert_array_equal(e.moments(), r["site_vals"])
+    e.close()
+
+
 @pytest.mark.parametrize("records,site_tcap", [(-1, 0), (1, 0), (700, 0), (700, 10)])
 def test_site_form_volume_records(vdn, O, records, site_tcap):
     """The site-major form hands finished polytopes to a separate volume pass
