@@ -5,6 +5,7 @@
 //   kn_perm.u32    kn_get_permutation()
 //   kn_knn.u32     kn_get_knearests()    (n x _K_, grid positions)
 //   dev_knn.i32    knn_cuda_global_dev(points, points, _K_ + 1) rows (k x n, self included)
+// and prints kn_print_stats() on stderr.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -36,6 +37,7 @@ int main(int argc, char** argv) {
 
   kn_problem* kn = kn_prepare(reinterpret_cast<float3*>(pts.data()), n);
   kn_solve(kn);
+  kn_print_stats(kn);
   float3* sp = kn_get_points(kn);
   unsigned int* perm = kn_get_permutation(kn);
   unsigned int* knn = kn_get_knearests(kn);

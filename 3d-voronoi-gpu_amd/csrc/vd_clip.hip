@@ -131,6 +131,28 @@ struct PlaneStoreRebuild {
   __device__ __forceinline__ void put(int i, const P4& p) { face[i * 64 + lane] = make_float4(p.x, p.y, p.z, p.w); }
 };
 
+// PlaneStoreTable: as PlaneStoreRebuild, but the seed's first `depth`
+// bisectors come from a table in LDS that the wave built once for the seed
+// (tbl null: none), with the same function and inputs, hence the same bits.
+struct PlaneStoreTable {
+  float4* face;          // [4][64] per lane
+  const float4* tbl;     // bisectors 0 .. depth-1 of the lane's seed, or null
+  int depth;
+  const float4* sites;
+  const int32_t* nbr;    // the seed's neighbour row
+  int n_sites;
+  P4 seed;
+  int lane;
+  __device__ __forceinline__ P4 get(int i) const {
+    if (i < 4) return ld4(face, i * 64 + lane);
+    if (tbl && i - 4 < depth) return ld4(tbl, i - 4);
+    const int z = nbr[i - 4];
+    if ((unsigned)z >= (unsigned)n_sites) return {0.f, 0.f, 0.f, 0.f};  // as the table marks it
+    return bisector(seed, ld4(sites, z));
+  }
+  __device__ __forceinline__ void put(int i, const P4& p) { face[i * 64 + lane] = make_float4(p.x, p.y, p.z, p.w); }
+};
+
 // Triangle (= polytope vertex) storage policies.
 // TriPlain: the vertex's plane triple only (the reference's layout); the
 //   security-radius test recomputes every vertex.
@@ -1419,19 +1441,30 @@ __device__ __forceinline__ void accumulate_wave(const ClipArgs& a, bool acc, int
   }
 }
 
+// Bisector tables of the volume pass: per wave, up to kMomentsTables seeds
+// of its 64 records (records come in runs of one seed) with their first
+// kMomentsDepth bisectors; the records of further seeds rebuild theirs.
+constexpr int kMomentsTables = 4;
+constexpr int kMomentsDepth = 48;
+
 template <int MF, int ACC>
 __global__ __launch_bounds__(kMomentsBlock) void k_site_moments(ClipArgs a) {
   __shared__ float4 face_all[kMomentsBlock / 64][4][64];
+  __shared__ float4 tbl_all[kMomentsBlock / 64][kMomentsTables][kMomentsDepth];
   __shared__ unsigned hist[8];
   const int lane = threadIdx.x & 63;
   if (threadIdx.x < 8) hist[threadIdx.x] = 0;
   __syncthreads();
-  Cell<PlaneStoreRebuild, TriRecord> c;
+  Cell<PlaneStoreTable, TriRecord> c;
   c.ps.face = &face_all[threadIdx.x >> 6][0][0];
   c.ps.sites = a.sites;
   c.ps.n_sites = a.n_sites;
   c.ps.lane = lane;
   c.keep_srnr = false;
+  float4(*tbl)[kMomentsDepth] = tbl_all[threadIdx.x >> 6];
+  // records hold bisectors below the launch's table depth only
+  const int depth = min(min(a.site_depth > 0 ? a.site_depth : a.K, a.K), kMomentsDepth);
+  c.ps.depth = depth;
   const unsigned n = min(*a.rec_count, a.rec_cap);
   const int stride = rec_stride(a.tcap);
   // wave-uniform loop: every lane takes part in the accumulation's shuffles
@@ -1443,6 +1476,27 @@ __global__ __launch_bounds__(kMomentsBlock) void k_site_moments(ClipArgs a) {
     const bool rec = m.x != ~0u;  // a reserved slot left unused, or past the end
     const unsigned pair = m.x;
     const int s = rec ? (int)m.y : 0;
+    // the bisector tables of the wave's first kMomentsTables seeds
+    int slot = -1;
+    {
+      unsigned long long todo = __ballot(rec);
+      for (int t = 0; t < kMomentsTables && todo; ++t) {
+        const int leader = __builtin_ctzll(todo);
+        const int ks = __shfl(s, leader);
+        const bool mine = rec && s == ks;
+        todo &= ~__ballot(mine);
+        if (mine) slot = t;
+        if (lane < depth) {
+          const int z = a.site_knn[(size_t)ks * a.K + lane];
+          const P4 p = (unsigned)z >= (unsigned)a.n_sites ? P4{0.f, 0.f, 0.f, 0.f}
+                                                          : bisector(site_at(a, ks), site_at(a, z));
+          tbl[t][lane] = make_float4(p.x, p.y, p.z, p.w);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
     PairSums<ACC> S;
     int status = kNoIntersection;
     bool acc = false;
@@ -1451,6 +1505,7 @@ __global__ __launch_bounds__(kMomentsBlock) void k_site_moments(ClipArgs a) {
       tet_corners(a, (int)m.z, v);
       c.seed = site_at(a, s);
       c.ps.seed = c.seed;
+      c.ps.tbl = slot >= 0 ? tbl[slot] : nullptr;
       c.ps.nbr = a.site_knn + (size_t)s * a.K;
       c.put_tet_faces(v);
       c.ts.tr = a.rec_tris + (size_t)i * stride;
@@ -1489,6 +1544,10 @@ __global__ __launch_bounds__(kMomentsBlock) void k_site_moments(ClipArgs a) {
       }
     }
     accumulate_wave<ACC>(a, acc, s, S);
+    // every lane is done with the tables before the next records' are built
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   __syncthreads();
   if (a.status_hist && threadIdx.x < 7 && hist[threadIdx.x])

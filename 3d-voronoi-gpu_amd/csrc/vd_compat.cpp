@@ -7,6 +7,7 @@
 // All of them run on the engine's kernels; none exits the process.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -89,6 +90,32 @@ struct GridTmp {
   }
 };
 
+// Arguments of vd_knn_dev (knncuda.cu:271-349: pitched
+// SoA rows, dim 3; the index pitch must hold every query).
+bool knn_dev_args_ok(const float* ref_dev, int ref_nb, size_t ref_pitch, const float* query_dev, int query_nb,
+                     size_t query_pitch, int dim, int k, const int* knn_index_dev, size_t index_pitch) {
+  return !(dim != 3 || k < 1 || k > 256 || ref_nb < 1 || query_nb < 0 || !ref_dev || !knn_index_dev ||
+           ref_pitch < (size_t)ref_nb || (query_nb && (!query_dev || query_pitch < (size_t)query_nb)) ||
+           index_pitch < (size_t)query_nb);
+}
+
+// The kNN of vd_knn_dev on given device buffers: points to float4, grid over
+// the reference points, one query per query point.  Enqueues only.
+hipError_t knn_dev_enqueue(const vd::GridBufs& gb, const int4* offsets, int n_offsets, float4* ref4, float4* q4,
+                           const float* ref_dev, int ref_nb, size_t ref_pitch, const float* query_dev, int query_nb,
+                           size_t query_pitch, int k, int* knn_index_dev, size_t index_pitch, hipStream_t st) {
+  hipLaunchKernelGGL(k_soa_to_f4, dim3((ref_nb + 255) / 256), dim3(256), 0, st, ref_dev, ref_pitch, ref_nb, ref4);
+  if (query_nb)
+    hipLaunchKernelGGL(k_soa_to_f4, dim3((query_nb + 255) / 256), dim3(256), 0, st, query_dev, query_pitch, query_nb,
+                       q4);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = vd::grid_build(gb, ref4, ref_nb, nullptr, nullptr, 3.1f, st);
+  if (e == hipSuccess)
+    e = vd::knn_query(gb, ref_nb, offsets, n_offsets, k, 2, query_nb, nullptr, nullptr, q4, nullptr, knn_index_dev,
+                      (long long)index_pitch, st);
+  return e;
+}
+
 }  // namespace
 
 struct vd_kn {
@@ -107,9 +134,8 @@ extern "C" {
 
 int vd_knn_dev(const float* ref_dev, int ref_nb, size_t ref_pitch, const float* query_dev, int query_nb,
                size_t query_pitch, int dim, int k, int* knn_index_dev, size_t index_pitch, void* stream) {
-  if (dim != 3 || k < 1 || k > 256 || ref_nb < 1 || query_nb < 0 || !ref_dev || !knn_index_dev ||
-      ref_pitch < (size_t)ref_nb || (query_nb && (!query_dev || query_pitch < (size_t)query_nb)) ||
-      index_pitch < (size_t)query_nb)
+  if (!knn_dev_args_ok(ref_dev, ref_nb, ref_pitch, query_dev, query_nb, query_pitch, dim, k, knn_index_dev,
+                       index_pitch))
     return VD_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   GridTmp g;
@@ -117,14 +143,8 @@ int vd_knn_dev(const float* ref_dev, int ref_nb, size_t ref_pitch, const float* 
   hipError_t e;
   if ((e = g.alloc(ref_nb, 3.1f)) || (e = ref4.alloc(ref_nb)) || (e = q4.alloc(query_nb)))
     return e == hipErrorOutOfMemory ? VD_ERR_NOMEM : VD_ERR_HIP;
-  hipLaunchKernelGGL(k_soa_to_f4, dim3((ref_nb + 255) / 256), dim3(256), 0, st, ref_dev, ref_pitch, ref_nb, ref4.p);
-  if (query_nb)
-    hipLaunchKernelGGL(k_soa_to_f4, dim3((query_nb + 255) / 256), dim3(256), 0, st, query_dev, query_pitch, query_nb,
-                       q4.p);
-  const vd::GridBufs gb = g.bufs();
-  if ((e = vd::grid_build(gb, ref4.p, ref_nb, nullptr, nullptr, 3.1f, st))) return VD_ERR_HIP;
-  if ((e = vd::knn_query(gb, ref_nb, g.offsets.p, g.n_offsets, k, 2, query_nb, nullptr, nullptr, q4.p, nullptr,
-                         knn_index_dev, (long long)index_pitch, st)))
+  if (knn_dev_enqueue(g.bufs(), g.offsets.p, g.n_offsets, ref4.p, q4.p, ref_dev, ref_nb, ref_pitch, query_dev,
+                      query_nb, query_pitch, k, knn_index_dev, index_pitch, st))
     return VD_ERR_HIP;
   if ((e = hipStreamSynchronize(st))) return VD_ERR_HIP;
   return VD_OK;
@@ -221,6 +241,12 @@ int vd_kn_get_knearests(vd_kn* kn, unsigned int* out) {
              : VD_OK;
 }
 
+int vd_kn_get_cell_counts(vd_kn* kn, int* out) {
+  if (!kn || !out) return VD_ERR_ARG;
+  const size_t cells = (size_t)kn->shape.dim[0] * kn->shape.dim[1] * kn->shape.dim[2];
+  return hipMemcpy(out, kn->grid.count.p, sizeof(int) * cells, hipMemcpyDeviceToHost) ? VD_ERR_HIP : VD_OK;
+}
+
 int vd_kn_get_permutation(vd_kn* kn, unsigned int* out) {
   if (!kn || !out) return VD_ERR_ARG;
   return hipMemcpy(out, kn->grid.ids.p, sizeof(unsigned int) * (size_t)kn->n, hipMemcpyDeviceToHost) ? VD_ERR_HIP
@@ -264,10 +290,14 @@ int vd_compute_clipped_voro_diagram(const float* vertices, int n_verts, const in
     if (rc != VD_OK) break;
     int K = 0, TK = 0;
     vd_get_dims(ctx, &K, &TK, nullptr, nullptr);
-    // columns as in voronoi.cu:985-1175, in seconds; the cell->tet column is
-    // not produced by this engine's iteration (0)
+    // columns as in voronoi.cu:985-1175, in seconds.  The reference runs both
+    // formulations every iteration and times kernel A under Clip_tet-cell and
+    // kernels B+C under Clip_cell-tet (voronoi.cu:1034-1116); this engine runs
+    // the configured one, whose time goes under its column, and 0 under the other
+    const double clip_s = (tm.pairs_ms + tm.clip_ms) / 1e3;
+    const bool cells = cfg.form == VD_FORM_CELLS;
     fprintf(rec, "%d, %d, %d, %d, %d, %.5f, %.5f, %.5f, %.5f, %.5f, \n", n_site, n_tets, TK, cfg.site_k, it,
-            (tm.knn_ms) / 1e3, (tm.pairs_ms + tm.clip_ms) / 1e3, 0.0, tm.update_ms / 1e3,
+            (tm.knn_ms) / 1e3, cells ? 0.0 : clip_s, cells ? clip_s : 0.0, tm.update_ms / 1e3,
             std::chrono::duration<double>(t2 - t1).count());
     printf("Iteration: %gs\n", std::chrono::duration<double>(t2 - t0).count());
     FILE* f = fopen((dir + "iter_" + std::to_string(it) + ".xyz").c_str(), "w");

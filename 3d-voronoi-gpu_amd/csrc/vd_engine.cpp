@@ -72,6 +72,25 @@ namespace vdi {
 // still lets 9 waves share a CU's 160 KB: 17 at 40 bisectors.  A batch whose
 // instances have fewer neighbours has smaller tables and the same cap.
 constexpr int kSiteWavesPerCu = 9;
+double fixed_point_load(const float* verts_xyz, int v0, int nv, const int32_t* tets, long long t0, int nt) {
+  double m = 0.0, vol = 0.0;
+  for (long long i = 3LL * v0; i < 3LL * (v0 + (long long)nv); ++i) m = std::max(m, std::fabs((double)verts_xyz[i]));
+  for (long long t = t0; t < t0 + nt; ++t) {
+    const float* p[4];
+    for (int k = 0; k < 4; ++k) p[k] = verts_xyz + 3 * (size_t)(v0 + tets[4 * t + k]);
+    double a[3], b[3], c[3];
+    for (int x = 0; x < 3; ++x) {
+      a[x] = (double)p[1][x] - p[0][x];
+      b[x] = (double)p[2][x] - p[0][x];
+      c[x] = (double)p[3][x] - p[0][x];
+    }
+    vol += std::fabs(a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) +
+                     a[2] * (b[0] * c[1] - b[1] * c[0])) /
+           6.0;
+  }
+  return vol * m;
+}
+
 int site_tcap_of(const vd_config& c) {
   if (c.site_tcap > 0) return std::min(c.site_tcap, c.tcap);
   const int depth = std::max(site_depth_of(c, c.site_k), 1);
@@ -105,6 +124,8 @@ struct vd_ctx {
   DevBuf<float> b_box;
   bool have_knn = false, have_clip = false;
   float mesh_lo[3] = {0, 0, 0}, mesh_hi[3] = {0, 0, 0};
+  // largest vdi::fixed_point_load of the mesh (or of the batch's instances)
+  double fixed_load = 0.0;
 
   DevBuf<float4> verts, sites;
   DevBuf<float4> tet_xyz;  // per tet its 12 corner coordinates (3 float4)
@@ -119,6 +140,11 @@ struct vd_ctx {
   int max_cells = 0;
   // kNN results
   DevBuf<int32_t> site_knn, tet_knn;
+  // site_knn / tet_knn hold a complete result (rows of rows_K / rows_TK
+  // distinct ids) of the current mesh, batch layout and site count: the next
+  // kNN step bounds each query by its previous row
+  bool rows_valid = false;
+  int rows_K = 0, rows_TK = 0;
   // site-major pairs
   DevBuf<int> p_count, p_cursor;
   DevBuf<unsigned> p_start, p_ids;
@@ -233,6 +259,9 @@ int check_config(vd_ctx* ctx, const vd_config& c) {
     return ctx->fail(VD_ERR_ARG, "srnr_k must be 0 or in (site_k, 256], got %d", c.srnr_k);
   if (c.form == VD_FORM_SITE_MAJOR && c.site_k + 4 > 254)
     return ctx->fail(VD_ERR_ARG, "site-major form needs site_k <= 250");
+  if (c.knn_waves != 0 && c.knn_waves != 2 && c.knn_waves != 4 && c.knn_waves != 8)
+    return ctx->fail(VD_ERR_ARG, "knn_waves must be 0, 2, 4 or 8, got %d", c.knn_waves);
+  if (c.clip_blocks < 0) return ctx->fail(VD_ERR_ARG, "clip_blocks must be >= 0, got %d", c.clip_blocks);
   return VD_OK;
 }
 
@@ -327,6 +356,7 @@ void vd_config_default(vd_config* c) {
   c->filter = 0;
   c->site_records = 0;
   c->site_depth = 0;
+  c->knn_waves = 0;
 }
 
 int vd_ctx_create(vd_ctx** out, int device, const vd_config* cfg) {
@@ -453,6 +483,7 @@ int load_mesh(vd_ctx* ctx, const float* verts_xyz, int n_verts, const int32_t* t
   ctx->n_verts = n_verts;
   ctx->n_tets = n_tets;
   ctx->have_knn = ctx->have_clip = false;
+  ctx->rows_valid = false;
   return VD_OK;
 }
 }  // namespace
@@ -466,7 +497,9 @@ int vd_set_mesh(vd_ctx* ctx, const float* verts_xyz, int n_verts, const int32_t*
       return ctx->fail(VD_ERR_PRECONDITION, "tet %lld refers to vertex %d of %d", i / 4, tets[i], n_verts);
   ctx->drop_graph();
   ctx->inst.clear();
-  return load_mesh(ctx, verts_xyz, n_verts, tets, n_tets);
+  const int rc = load_mesh(ctx, verts_xyz, n_verts, tets, n_tets);
+  if (rc == VD_OK) ctx->fixed_load = vdi::fixed_point_load(verts_xyz, 0, n_verts, tets, 0, n_tets);
+  return rc;
 }
 
 int vd_set_batch(vd_ctx* ctx, int n_inst, const int32_t* n_verts, const int32_t* n_tets, const int32_t* n_sites,
@@ -513,7 +546,11 @@ int vd_set_batch(vd_ctx* ctx, int n_inst, const int32_t* n_verts, const int32_t*
     for (int k = q.t0; k < q.t0 + q.nt; ++k) tet_inst[k] = i;
   }
   site_begin[n_inst] = (int)S;
+  double fixed_load = 0.0;
+  for (int i = 0; i < n_inst; ++i)
+    fixed_load = std::max(fixed_load, vdi::fixed_point_load(verts_xyz, in[i].v0, in[i].nv, tets, in[i].t0, in[i].nt));
   int rc = load_mesh(ctx, verts_xyz, (int)V, gt.data(), (int)T);
+  ctx->fixed_load = fixed_load;
   if (rc != VD_OK) return rc;
   rc = upload_xyz(ctx, ctx->sites, sites_xyz, (int)S);
   if (rc != VD_OK) return rc;
@@ -549,7 +586,10 @@ int vd_set_sites(vd_ctx* ctx, const float* sites_xyz, int n_sites) {
   VD_HIP(hipSetDevice(ctx->device), "set device");
   if (ctx->batch() && n_sites != ctx->n_sites)
     return ctx->fail(VD_ERR_STATE, "vd_set_sites: a batch holds %d sites, got %d", ctx->n_sites, n_sites);
-  if (n_sites != ctx->n_sites) ctx->drop_graph();
+  if (n_sites != ctx->n_sites) {
+    ctx->drop_graph();
+    ctx->rows_valid = false;
+  }
   if (!ctx->batch()) ctx->inst.clear();
   int rc = upload_xyz(ctx, ctx->sites, sites_xyz, n_sites);
   if (rc != VD_OK) return rc;
@@ -619,21 +659,27 @@ int vd_knn_step(vd_ctx* ctx) {
   // the GPU idle, and the step ends when both have.
   const bool tets = ctx->n_tets > 0 && ctx->TK > 0;
   const bool fork = ctx->K > 0 && tets;
+  // the rows of the last kNN step bound this one's queries (same points, moved)
+  const float4* hint = ctx->rows_valid && ctx->rows_K == ctx->K && ctx->rows_TK == ctx->TK ? ctx->sites.p : nullptr;
   if (fork) {
     VD_HIP(hipEventRecord(ctx->fork_ev, ctx->stream), "event");
     VD_HIP(hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0), "stream wait");
   }
   if (ctx->K > 0)
     VD_HIP(vd::knn_query(gb, n, ctx->offsets.p, ctx->n_offsets, ctx->K, 0, n, nullptr, nullptr, nullptr, nullptr,
-                         ctx->site_knn.p, 0, fork ? ctx->side : ctx->stream),
+                         ctx->site_knn.p, 0, fork ? ctx->side : ctx->stream, hint, ctx->cfg.knn_waves),
            "site knn");
   if (fork) VD_HIP(hipEventRecord(ctx->join_ev, ctx->side), "event");
   if (tets)
     VD_HIP(vd::knn_query(gb, n, ctx->offsets.p, ctx->n_offsets, ctx->TK, 1, ctx->n_tets, ctx->verts.p, ctx->tets.p,
-                         nullptr, ctx->batch() ? ctx->b_tet_inst.p : nullptr, ctx->tet_knn.p, 0, ctx->stream),
+                         nullptr, ctx->batch() ? ctx->b_tet_inst.p : nullptr, ctx->tet_knn.p, 0, ctx->stream, hint,
+                         ctx->cfg.knn_waves),
            "tet knn");
   if (fork) VD_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0), "stream wait");
   VD_HIP(record_phase(ctx, (1)), "event");
+  ctx->rows_valid = true;
+  ctx->rows_K = ctx->K;
+  ctx->rows_TK = ctx->TK;
   ctx->ev_knn = true;
   ctx->ev_pairs = ctx->ev_clip = ctx->ev_update = false;
   ctx->have_knn = true;
@@ -649,6 +695,11 @@ int vd_clip_step(vd_ctx* ctx) {
   const int n = ctx->n_sites;
   const long long n_pairs = (long long)ctx->n_tets * ctx->TK;
   if (n_pairs > 0xffffffffLL) return ctx->fail(VD_ERR_ARG, "more than 2^32 - 1 pairs in one context");
+  if (c.accum == VD_ACCUM_F64 && ctx->fixed_load > vdi::kFixedPointMaxLoad)
+    return ctx->fail(VD_ERR_PRECONDITION,
+                     "mesh volume x largest |coordinate| = %.3g exceeds %.3g, the range of the exact FP64 "
+                     "fixed-point sums: map the mesh into the [0,1000]^3 frame or use VD_ACCUM_F32",
+                     ctx->fixed_load, vdi::kFixedPointMaxLoad);
   if (c.accum == VD_ACCUM_F32) {
     VD_HIP(ctx->acc_f32.ensure(4 * (size_t)n), "alloc acc");
     VD_HIP(vd::fill_u32(ctx->acc_f32.p, 4LL * n, 0u, ctx->stream), "clear acc");
@@ -970,8 +1021,10 @@ int vd_autotune(vd_ctx* ctx, int iters, vd_config* chosen) {
   const int tcaps[] = {12, 16, 20, 24};
   const int depths[] = {8, 16, 32, 64};
   const float densities[] = {2.0f, 3.1f, 5.0f};
-  for (int dim = 0; dim < 3; ++dim) {
-    const int nv = dim == 2 ? 3 : 4;
+  const int clip_blocks[] = {1536, 3072, 4608};  // 0: as many as stay resident
+  const int knn_waves[] = {2, 4, 8};
+  for (int dim = 0; dim < 5; ++dim) {
+    const int nv = dim == 2 || dim >= 3 ? 3 : 4;
     for (int j = 0; j < nv; ++j) {
       vd_config c = best;
       if (dim == 0) {
@@ -980,8 +1033,13 @@ int vd_autotune(vd_ctx* ctx, int iters, vd_config* chosen) {
       } else if (dim == 1) {
         if (c.form != VD_FORM_SITE_MAJOR) break;
         c.cull_depth = depths[j];
-      } else {
+      } else if (dim == 2) {
         c.grid_density = densities[j];
+      } else if (dim == 3) {
+        if (c.form != VD_FORM_SITE_MAJOR) break;
+        c.clip_blocks = clip_blocks[j];
+      } else {
+        c.knn_waves = knn_waves[j];
       }
       if (std::memcmp(&c, &best, sizeof c) == 0) continue;
       double ms = 0.0;
@@ -996,6 +1054,97 @@ int vd_autotune(vd_ctx* ctx, int iters, vd_config* chosen) {
   if ((rc = restore()) != VD_OK) return rc;
   VD_HIP(vd_timing_reset(ctx) == VD_OK ? hipSuccess : hipErrorUnknown, "timing reset");
   if (chosen) *chosen = best;
+  return VD_OK;
+}
+
+int vd_autotune_k(vd_ctx* ctx, int iters, double tol, vd_config* chosen, double* err_out) {
+  if (!ctx || iters < 1 || !(tol >= 0.0)) return VD_ERR_ARG;
+  if (ctx->n_sites <= 0) return ctx->fail(VD_ERR_STATE, "vd_autotune_k: no sites");
+  VD_HIP(hipSetDevice(ctx->device), "set device");
+  const int n = ctx->n_sites;
+  DevBuf<float4> saved;
+  VD_HIP(saved.ensure(n), "alloc");
+  VD_HIP(hipMemcpyAsync(saved.p, ctx->sites.p, sizeof(float4) * n, hipMemcpyDeviceToDevice, ctx->stream),
+         "save sites");
+  VD_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  std::vector<float> start(3 * (size_t)n), ref(3 * (size_t)n), got(3 * (size_t)n);
+  int rc = vd_get_sites(ctx, start.data());
+  if (rc != VD_OK) return rc;
+  auto restore = [&]() -> int {
+    VD_HIP(hipMemcpyAsync(ctx->sites.p, saved.p, sizeof(float4) * n, hipMemcpyDeviceToDevice, ctx->stream),
+           "restore sites");
+    VD_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+    ctx->have_knn = ctx->have_clip = false;
+    return VD_OK;
+  };
+  // iters iterations under cfg from the saved sites: device ms per iteration
+  // (after the first, or of the first when iters == 1) and the final sites
+  auto trial = [&](const vd_config& cfg, double* ms, std::vector<float>& out) -> int {
+    *ms = -1.0;
+    int r = vd_ctx_set_config(ctx, &cfg);
+    if (r == VD_OK) r = restore();
+    if (r == VD_OK) r = vd_timing_reset(ctx);
+    if (r == VD_OK) r = vd_lloyd(ctx, iters);
+    if (r != VD_OK) return r;
+    std::vector<vd_timing> t((size_t)iters);
+    if ((r = vd_get_timing_list(ctx, t.data(), iters)) != VD_OK) return r;
+    double sum = 0.0;
+    for (int i = iters > 1 ? 1 : 0; i < iters; ++i) sum += t[i].total_ms;
+    *ms = sum / (iters > 1 ? iters - 1 : 1);
+    return vd_get_sites(ctx, out.data());
+  };
+  // largest displacement relative to |site|, over all sites
+  auto error = [&](const std::vector<float>& a, const std::vector<float>& b) {
+    double e = 0.0;
+    for (int i = 0; i < n; ++i) {
+      double d2 = 0.0, r2 = 0.0;
+      for (int x = 0; x < 3; ++x) {
+        const double d = (double)a[3 * i + x] - b[3 * i + x];
+        d2 += d * d;
+        r2 += (double)b[3 * i + x] * b[3 * i + x];
+      }
+      if (d2 > 0.0) e = std::max(e, std::sqrt(d2 / std::max(r2, 1e-300)));
+    }
+    return e;
+  };
+  const vd_config base = ctx->cfg;
+  int K0 = 0, TK0 = 0;
+  vd_get_dims(ctx, &K0, &TK0, nullptr, nullptr);
+  if (TK0 < 0) return ctx->fail(VD_ERR_ARG, "vd_autotune_k: the batch's instances need one tet_k");
+  vd_config best = base;
+  best.site_k = std::max(K0, 1);
+  best.tet_k = std::max(TK0, 1);
+  double best_ms = 0.0, best_err = 0.0;
+  if ((rc = trial(best, &best_ms, ref)) != VD_OK) return rc;
+  const double fr[] = {0.875, 0.75, 0.625, 0.5, 0.4, 0.3};
+  for (int dim = 0; dim < 2; ++dim) {
+    for (double f : fr) {
+      vd_config c = best;
+      if (dim == 0) {
+        c.tet_k = std::max(8, (int)std::lround(f * TK0));
+        if (c.tet_k >= best.tet_k) continue;
+      } else {
+        c.site_k = std::max(16, (int)std::lround(f * K0));
+        if (c.site_k >= best.site_k) continue;
+        if (c.srnr_k != 0 && c.srnr_k <= c.site_k) c.srnr_k = 0;
+      }
+      if (check_config(ctx, c) != VD_OK) break;
+      double ms = 0.0;
+      if ((rc = trial(c, &ms, got)) != VD_OK) return rc;
+      const double e = error(got, ref);
+      if (e > tol) break;  // smaller counts only move the sites further
+      if (ms > 0.0 && ms < best_ms) {
+        best_ms = ms;
+        best = c;
+        best_err = e;
+      }
+    }
+  }
+  if ((rc = vd_ctx_set_config(ctx, &best)) != VD_OK) return rc;
+  if ((rc = restore()) != VD_OK) return rc;
+  if ((rc = vd_timing_reset(ctx)) != VD_OK) return rc;
+  if (chosen) *chosen = best;
+  if (err_out) *err_out = best_err;
   return VD_OK;
 }
 
@@ -1060,6 +1209,19 @@ int vd_get_timing_sum(vd_ctx* ctx, vd_timing* t, int* n_iters) {
   return VD_OK;
 }
 
+int vd_get_timing_list(vd_ctx* ctx, vd_timing* t, int n) {
+  if (!ctx || n < 0 || (n && !t)) return VD_ERR_ARG;
+  if (n > std::min<long long>(ctx->it_done, vd_ctx::kRing))
+    return ctx->fail(VD_ERR_ARG, "vd_get_timing_list: %d iterations asked, %lld kept", n,
+                     std::min<long long>(ctx->it_done, vd_ctx::kRing));
+  VD_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+  for (int i = 0; i < n; ++i) {
+    std::memset(&t[i], 0, sizeof t[i]);
+    slot_timing(ctx, (ctx->it_slot + vd_ctx::kRing - n + i) % vd_ctx::kRing, true, true, true, true, &t[i]);
+  }
+  return VD_OK;
+}
+
 int vd_get_site_knn(vd_ctx* ctx, int32_t* out) {
   if (!ctx || !out) return VD_ERR_ARG;
   if (!ctx->have_knn && !ctx->have_clip) return ctx->fail(VD_ERR_STATE, "no kNN result");
@@ -1103,6 +1265,37 @@ int vd_get_pairs(vd_ctx* ctx, uint8_t* status, double* vals) {
     VD_HIP(hipMemcpyAsync(vals, ctx->pair_vals.p, 4 * sizeof(double) * np, hipMemcpyDeviceToHost, ctx->stream),
            "read pairs");
   VD_HIP(hipStreamSynchronize(ctx->stream), "read pairs");
+  return VD_OK;
+}
+
+int vd_get_pairs_range(vd_ctx* ctx, long long first, long long count, uint8_t* status, double* vals) {
+  if (!ctx) return VD_ERR_ARG;
+  if (!ctx->cfg.record_pairs) return ctx->fail(VD_ERR_STATE, "record_pairs is off");
+  if (!ctx->have_clip && !ctx->ev_update) return ctx->fail(VD_ERR_STATE, "no clip result");
+  const long long np = (long long)ctx->n_tets * ctx->TK;
+  if (first < 0 || count < 0 || first > np || count > np - first)
+    return ctx->fail(VD_ERR_ARG, "pair range [%lld, %lld) outside [0, %lld)", first, first + count, np);
+  if (status && count)
+    VD_HIP(hipMemcpyAsync(status, ctx->pair_status.p + first, (size_t)count, hipMemcpyDeviceToHost, ctx->stream),
+           "read pairs");
+  if (vals && count)
+    VD_HIP(hipMemcpyAsync(vals, ctx->pair_vals.p + 4 * first, 4 * sizeof(double) * (size_t)count,
+                          hipMemcpyDeviceToHost, ctx->stream),
+           "read pairs");
+  VD_HIP(hipStreamSynchronize(ctx->stream), "read pairs");
+  return VD_OK;
+}
+
+int vd_get_tet_knn_range(vd_ctx* ctx, int first_tet, int n_tets, int32_t* out) {
+  if (!ctx || (!out && n_tets)) return VD_ERR_ARG;
+  if (!ctx->have_knn && !ctx->have_clip) return ctx->fail(VD_ERR_STATE, "no kNN result");
+  if (first_tet < 0 || n_tets < 0 || first_tet > ctx->n_tets || n_tets > ctx->n_tets - first_tet)
+    return ctx->fail(VD_ERR_ARG, "tet range [%d, %d) outside [0, %d)", first_tet, first_tet + n_tets, ctx->n_tets);
+  if ((size_t)n_tets * ctx->TK)
+    VD_HIP(hipMemcpyAsync(out, ctx->tet_knn.p + (size_t)first_tet * ctx->TK, sizeof(int32_t) * (size_t)n_tets * ctx->TK,
+                          hipMemcpyDeviceToHost, ctx->stream),
+           "read tet knn");
+  VD_HIP(hipStreamSynchronize(ctx->stream), "read tet knn");
   return VD_OK;
 }
 
@@ -1181,6 +1374,9 @@ int assemble(vd_ctx* ctx, const JobDev* jobs, int n) {
   ctx->n_tets = (int)T;
   ctx->n_sites = (int)S;
   ctx->inst = in;
+  ctx->rows_valid = false;
+  ctx->fixed_load = 0.0;
+  for (int i = 0; i < n; ++i) ctx->fixed_load = std::max(ctx->fixed_load, jobs[i].fixed_load);
   if (n > 1) {
     std::vector<int> site_begin((size_t)n + 1), voff((size_t)n);
     std::vector<float> box((size_t)6 * n);

@@ -286,6 +286,12 @@ struct QueryArgs {
   const float4* qpts;
   int32_t* out;        // row-major n_query x k, or pitched rows when pitch > 0
   long long out_pitch; // > 0: out[j * pitch + q] (column per query)
+  // modes 0 and 1 with row-major output: when hint_pts is given, every row of
+  // `out` holds a previous result of this problem (k distinct reference ids of
+  // the query's instance, the query itself excluded) and hint_pts holds the
+  // reference points by original id
+  const float4* hint_pts;
+  const int* pt_begin;  // first point of every instance (n_inst + 1), or null
 };
 
 template <int M>
@@ -354,6 +360,54 @@ __device__ __forceinline__ uint64_t topk_kth(const TopK<M>& L, int k) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Inserts one key (wave-uniform) into the sorted list: the keys after its
+// place move up one slot and the largest drops out.
+template <int M>
+__device__ __forceinline__ void topk_insert(TopK<M>& L, uint64_t x, int lane) {
+  int pos = 0;
+#pragma unroll
+  for (int s = 0; s < M; ++s) pos += __popcll(__ballot(L.s[s] < x));
+  if (pos >= 64 * M) return;
+  uint64_t carry = x;  // last key of the previous slot (x before slot 0)
+#pragma unroll
+  for (int s = 0; s < M; ++s) {
+    const uint64_t v = L.s[s];
+    const uint64_t up = bperm64(v, lane == 0 ? 0 : lane - 1);
+    const int e = s * 64 + lane;
+    const uint64_t prev = lane == 0 ? carry : up;
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+    carry = ((uint64_t)hi << 32) | lo;
+    L.s[s] = e < pos ? v : (e == pos ? x : prev);
+  }
+}
+
+// Largest squared distance from qp to the k points of a previous result row
+// (hint_pts by id), or -1 when the row is not k distinct points of the
+// instance [lo, hi) other than the query itself.  Any k such points bound the
+// query's k-th squared distance from above.
+template <int M>
+__device__ __forceinline__ float hint_bound(const QueryArgs& a, int out_row, const float4& qp, int self, int lo,
+                                            int hi, int lane) {
+  float mx = 0.f;
+  bool bad = false;
+#pragma unroll
+  for (int s = 0; s < M; ++s) {
+    const int j = s * 64 + lane;
+    if (j < a.k) {
+      const int id = a.out[(long long)out_row * a.k + j];
+      if (id < lo || id >= hi || id == self)
+        bad = true;
+      else
+        mx = fmaxf(mx, dist2(a.hint_pts[id], qp));
+    }
+  }
+  if (__ballot(bad) != 0) return -1.f;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+  return mx;
+}
+
 // Squared distance from q to the box of cell (cx,cy,cz), minus the slack,
 // scaled down so that any point binned in that cell has a larger FP32 d2.
 __device__ __forceinline__ float cell_bound(const GridShape& g, const float4& q, int cx, int cy, int cz) {
@@ -373,10 +427,10 @@ __device__ __forceinline__ float cell_bound(const GridShape& g, const float4& q,
   return s * (1.f - 1e-5f);
 }
 
-#ifndef VD_KNN_WAVES
-#define VD_KNN_WAVES 4
-#endif
-constexpr int kQueryWaves = VD_KNN_WAVES;
+// Waves per workgroup of the query kernel: 2, 4 or 8 (knn_query's `waves`);
+// each wave works alone, the workgroup size only changes how waves share a
+// CU's LDS and are scheduled.
+constexpr int kQueryWavesDefault = 4;
 
 // Consecutive queries (tets of one region, or sites of one grid cell) are
 // close to each other: a wave takes a run of kQueryRun of them, and each
@@ -402,6 +456,13 @@ constexpr int kQueryRun = VD_KNN_RUN;
 #define VD_KNN_UNION_CAP 384
 #endif
 constexpr int kUnionCap = VD_KNN_UNION_CAP;
+
+// At most this many keys staged after the last merge are inserted one at a
+// time instead of sorted and merged.
+#ifndef VD_KNN_INSERT_MAX
+#define VD_KNN_INSERT_MAX 12
+#endif
+constexpr int kInsertMax = VD_KNN_INSERT_MAX;
 
 // Point of query q, its excluded reference id (or -1), output row and instance.
 __device__ __forceinline__ void query_point(const QueryArgs& a, int q, float4& qp, int& self, int& out_row, int& inst) {
@@ -432,7 +493,7 @@ __device__ __forceinline__ void query_point(const QueryArgs& a, int q, float4& q
   }
 }
 
-template <int M>
+template <int M, int kQueryWaves>
 __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
   __shared__ uint64_t stage_all[kQueryWaves][128];
   __shared__ int mark_all[kQueryWaves][64];  // batch_owner's marks
@@ -452,6 +513,10 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
   int n_uni = -1, uni_inst = -1;
   float r0 = 0.f;
   float4 q0p = {0.f, 0.f, 0.f, 0.f};
+  // with previous results (hint_pts): lane j holds the hint bound of query
+  // q0 + j once it is known (-2 before; -1: that row gives none)
+  const bool use_hint = a.hint_pts && a.out_pitch <= 0 && (a.mode == 0 || a.mode == 1);
+  float run_hb = -2.f;
   for (int q = q0; q < q0 + kQueryRun && q < nq; ++q) {  // wave-uniform
 
   float4 qp;
@@ -479,6 +544,15 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
     const float b = r0 + sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
     const float b2 = b * b * (1.f + 1e-5f);
     bound = umin64(bound, ((uint64_t)__float_as_uint(b2) << 32) | 0xffffffffull);
+  }
+  if (use_hint) {
+    // the query's previous result: its k points' current distances
+    float hb = __shfl(run_hb, q - q0);
+    if (hb == -2.f) {
+      const int lo = a.pt_begin ? a.pt_begin[inst] : 0, hi = a.pt_begin ? a.pt_begin[inst + 1] : a.n_ref;
+      hb = hint_bound<M>(a, out_row, qp, self, lo, hi, lane);
+    }
+    if (hb >= 0.f) bound = umin64(bound, ((uint64_t)__float_as_uint(hb) << 32) | 0xffffffffull);
   }
   uint64_t thr = bound;
   int staged = 0;
@@ -583,7 +657,16 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
       push(key, ok);
     }
   }
-  if (staged > 0) {
+  if (staged > 0 && !empty && staged <= kInsertMax) {
+    // a few keys past the last merge: inserted one by one
+    for (int i = 0; i < staged; ++i) {
+      const uint64_t x = stage[i];
+      if (x < thr) {
+        topk_insert<M>(L, x, lane);
+        thr = umin64(bound, topk_kth<M>(L, a.k));
+      }
+    }
+  } else if (staged > 0) {
     const uint64_t x = lane < staged ? stage[lane] : ~0ull;
     topk_merge<M>(L, x, lane, empty);
   }
@@ -607,7 +690,7 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
   if (q == q0 && prev_r >= 0.f && q + 1 < q_end) {
     // the run's other queries: their points (lane j takes query q0 + j)
     float4 pj = qp;
-    int sj, rj, ij = inst;
+    int sj = self, rj = out_row, ij = inst;
     const int qj = q0 + lane;
     if (lane >= 1 && qj < q_end) query_point(a, qj, pj, sj, rj, ij);
     const bool in = lane >= 1 && qj < q_end;
@@ -623,11 +706,23 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
       }
       const float inv = 1.f / (float)n_in;
       const float4 c = make_float4(cx * inv, cy * inv, cz * inv, 0.f);
+      if (use_hint) {
+        // the other queries' bounds from their previous results (lane j: query q0 + j)
+        const int lo = a.pt_begin ? a.pt_begin[inst] : 0, hi = a.pt_begin ? a.pt_begin[inst + 1] : a.n_ref;
+        for (int j = 1; j <= n_in; ++j) {
+          const float4 qj4 = make_float4(__shfl(pj.x, j), __shfl(pj.y, j), __shfl(pj.z, j), 0.f);
+          const int sj_row = __shfl(rj, j), sj_self = __shfl(sj, j);
+          const float hb = hint_bound<M>(a, sj_row, qj4, sj_self, lo, hi, lane);
+          if (lane == j) run_hb = hb;
+        }
+      }
       float rj_need = 0.f;
       if (in) {
         const float ax = pj.x - c.x, ay = pj.y - c.y, az = pj.z - c.z;
         const float bx = pj.x - qp.x, by = pj.y - qp.y, bz = pj.z - qp.z;
-        rj_need = sqrtf(fmaf(az, az, fmaf(ay, ay, ax * ax))) + prev_r + sqrtf(fmaf(bz, bz, fmaf(by, by, bx * bx)));
+        float rq = prev_r + sqrtf(fmaf(bz, bz, fmaf(by, by, bx * bx)));
+        if (run_hb >= 0.f) rq = fminf(rq, sqrtf(run_hb));
+        rj_need = sqrtf(fmaf(az, az, fmaf(ay, ay, ax * ax))) + rq;
       }
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) rj_need = fmaxf(rj_need, __shfl_xor(rj_need, o));
@@ -871,11 +966,32 @@ hipError_t grid_build(const GridBufs& gb, const float4* pts, int n, const float*
   return hipGetLastError();
 }
 
+namespace {
+template <int W>
+hipError_t launch_query_w(const QueryArgs& a, int n_query, hipStream_t st) {
+  const dim3 grid((n_query + W * kQueryRun - 1) / (W * kQueryRun)), block(64 * W);
+  if (a.k <= 64)
+    hipLaunchKernelGGL((k_knn_query<1, W>), grid, block, 0, st, a);
+  else if (a.k <= 128)
+    hipLaunchKernelGGL((k_knn_query<2, W>), grid, block, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_knn_query<4, W>), grid, block, 0, st, a);
+  return hipGetLastError();
+}
+hipError_t launch_query(const QueryArgs& a, int n_query, int waves, hipStream_t st) {
+  if (waves == 2) return launch_query_w<2>(a, n_query, st);
+  if (waves == 8) return launch_query_w<8>(a, n_query, st);
+  return launch_query_w<kQueryWavesDefault>(a, n_query, st);
+}
+}  // namespace
+
 hipError_t knn_query(const GridBufs& gb, int n_ref, const int4* offsets, int n_offsets, int k, int mode,
                      int n_query, const float4* verts, const int4* tets, const float4* qpts, const int* q_inst,
-                     int32_t* out, long long out_pitch, hipStream_t st) {
+                     int32_t* out, long long out_pitch, hipStream_t st, const float4* hint_pts, int waves) {
   if (k < 1 || k > 256) return hipErrorInvalidValue;
   QueryArgs a;
+  a.hint_pts = hint_pts;
+  a.pt_begin = gb.pt_begin;
   a.grid = gb.shape;
   a.cell_base = gb.cell_base;
   a.pt_inst = gb.pt_inst;
@@ -897,14 +1013,7 @@ hipError_t knn_query(const GridBufs& gb, int n_ref, const int4* offsets, int n_o
   a.q_ids = nullptr;
   a.n_query_dev = nullptr;
   if (n_query <= 0) return hipSuccess;
-  const dim3 grid((n_query + kQueryWaves * kQueryRun - 1) / (kQueryWaves * kQueryRun)), block(64 * kQueryWaves);
-  if (k <= 64)
-    hipLaunchKernelGGL(k_knn_query<1>, grid, block, 0, st, a);
-  else if (k <= 128)
-    hipLaunchKernelGGL(k_knn_query<2>, grid, block, 0, st, a);
-  else
-    hipLaunchKernelGGL(k_knn_query<4>, grid, block, 0, st, a);
-  return hipGetLastError();
+  return launch_query(a, n_query, waves, st);
 }
 
 hipError_t knn_query_ids(const GridBufs& gb, const int4* offsets, int n_offsets, int k, const float4* pts,
@@ -912,6 +1021,8 @@ hipError_t knn_query_ids(const GridBufs& gb, const int4* offsets, int n_offsets,
   if (k < 1 || k > 256) return hipErrorInvalidValue;
   if (n_ids_max <= 0) return hipSuccess;
   QueryArgs a;
+  a.hint_pts = nullptr;
+  a.pt_begin = gb.pt_begin;
   a.grid = gb.shape;
   a.cell_base = gb.cell_base;
   a.pt_inst = gb.pt_inst;
@@ -932,14 +1043,7 @@ hipError_t knn_query_ids(const GridBufs& gb, const int4* offsets, int n_offsets,
   a.out_pitch = 0;
   a.q_ids = ids;
   a.n_query_dev = n_ids_dev;
-  const dim3 grid((n_ids_max + kQueryWaves * kQueryRun - 1) / (kQueryWaves * kQueryRun)), block(64 * kQueryWaves);
-  if (k <= 64)
-    hipLaunchKernelGGL(k_knn_query<1>, grid, block, 0, st, a);
-  else if (k <= 128)
-    hipLaunchKernelGGL(k_knn_query<2>, grid, block, 0, st, a);
-  else
-    hipLaunchKernelGGL(k_knn_query<4>, grid, block, 0, st, a);
-  return hipGetLastError();
+  return launch_query(a, n_ids_max, kQueryWavesDefault, st);
 }
 
 }  // namespace vd

@@ -70,10 +70,18 @@ hipError_t grid_build(const GridBufs& gb, const float4* pts, int n, const float*
 //   mode 2: queries are qpts[0..n_query).
 // out: row-major n_query x k (out_pitch == 0) or out[j*out_pitch + q].
 // Missing neighbours (k larger than the point count) are -1.  q_inst: the
-// instance of every query of mode 1 or 2 (null: instance 0).
+// instance of every query of mode 1 or 2 (null: instance 0).  hint_pts
+// (modes 0 and 1, row-major out): out already holds a complete result of this
+// problem for earlier positions of the same points (k distinct ids of the
+// query's instance per row, itself excluded), and hint_pts holds the points by
+// original id; the query's k-th distance is then bounded from the start by
+// those k points' current distances.  The result does not depend on it.
+// waves: wavefronts per workgroup, 2, 4 or 8 (other values: 4); the result
+// does not depend on it either.
 hipError_t knn_query(const GridBufs& gb, int n_ref, const int4* offsets, int n_offsets, int k, int mode,
                      int n_query, const float4* verts, const int4* tets, const float4* qpts, const int* q_inst,
-                     int32_t* out, long long out_pitch, hipStream_t st);
+                     int32_t* out, long long out_pitch, hipStream_t st, const float4* hint_pts = nullptr,
+                     int waves = 4);
 
 // k nearest binned points of the points pts[ids[q]], q < min(*n_ids_dev,
 // n_ids_max), each excluding itself; row q of out (row-major, k per row).

@@ -21,6 +21,8 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <iterator>
+#include <map>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -43,7 +45,8 @@ struct Job {
   std::vector<float> result;  // final sites (xyz) once done
   float box[6] = {};
   size_t bytes = 0;
-  void* block = nullptr;  // device buffers: verts, tets, tet records, sites
+  void* block = nullptr;  // device buffers: verts, tets, tet records, sites (from the arena)
+  size_t block_bytes = 0;
   vdi::JobDev dev{};
   Clock::time_point t_submit, t_admit, t_done;
 };
@@ -54,10 +57,85 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Device memory of the jobs' own buffers: chunks taken from the runtime as the
+// running set first needs them and kept until the server ends, handed out by
+// first fit with neighbouring free ranges merged on release, so that admitting
+// and retiring jobs allocates and frees nothing on the device (a hipFree
+// waits for the whole device).
+class Arena {
+ public:
+  static constexpr size_t kChunk = (size_t)1 << 30;
+
+  ~Arena() {
+    for (auto& c : chunks_) (void)hipFree(c.base);
+  }
+
+  // A 256-byte-aligned range of `bytes`, or nullptr when the device has no
+  // memory for a new chunk.
+  void* take(size_t bytes) {
+    bytes = align256(std::max<size_t>(bytes, 1));
+    for (auto& c : chunks_) {
+      for (auto it = c.free.begin(); it != c.free.end(); ++it) {
+        if (it->second < bytes) continue;
+        const size_t off = it->first, len = it->second;
+        c.free.erase(it);
+        if (len > bytes) c.free.emplace(off + bytes, len - bytes);
+        return c.base + off;
+      }
+    }
+    Chunk c;
+    c.size = std::max(kChunk, bytes);
+    if (hipMalloc((void**)&c.base, c.size) != hipSuccess) return nullptr;
+    if (c.size > bytes) c.free.emplace(bytes, c.size - bytes);
+    chunks_.push_back(std::move(c));
+    return chunks_.back().base;
+  }
+
+  void give(void* p, size_t bytes) {
+    if (!p) return;
+    bytes = align256(std::max<size_t>(bytes, 1));
+    for (auto& c : chunks_) {
+      char* q = (char*)p;
+      if (q < c.base || q >= c.base + c.size) continue;
+      size_t off = (size_t)(q - c.base), len = bytes;
+      auto next = c.free.lower_bound(off);
+      if (next != c.free.end() && next->first == off + len) {
+        len += next->second;
+        next = c.free.erase(next);
+      }
+      if (next != c.free.begin()) {
+        auto prev = std::prev(next);
+        if (prev->first + prev->second == off) {
+          off = prev->first;
+          len += prev->second;
+          c.free.erase(prev);
+        }
+      }
+      c.free.emplace(off, len);
+      return;
+    }
+  }
+
+  size_t reserved() const {
+    size_t n = 0;
+    for (const auto& c : chunks_) n += c.size;
+    return n;
+  }
+
+ private:
+  struct Chunk {
+    char* base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free;  // offset -> length
+  };
+  std::vector<Chunk> chunks_;
+};
+
 }  // namespace
 
 struct vd_server {
   vd_ctx* ctx = nullptr;
+  Arena arena;
   int device = 0;
   vd_config cfg{};
   size_t budget = 0, used = 0;
@@ -94,7 +172,8 @@ struct vd_server {
   int admit(Job& j) {
     const size_t bv = align256(16 * (size_t)std::max(j.nv, 1)), bt = align256(16 * (size_t)std::max(j.nt, 1)),
                  bx = align256(48 * (size_t)std::max(j.nt, 1)), bs = align256(16 * (size_t)j.ns);
-    if (hipSetDevice(device) != hipSuccess || hipMalloc(&j.block, bv + bt + bx + bs) != hipSuccess) {
+    j.block_bytes = bv + bt + bx + bs;
+    if (hipSetDevice(device) != hipSuccess || !(j.block = arena.take(j.block_bytes))) {
       j.block = nullptr;
       return fail(VD_ERR_NOMEM, "job buffers: device allocation failed");
     }
@@ -111,7 +190,7 @@ struct vd_server {
     ok = ok && hipMemcpy(b + bv + bt, xyz.data(), 48 * (size_t)j.nt, hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemcpy(b + bv + bt + bx, s4.data(), 16 * (size_t)j.ns, hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {
-      (void)hipFree(j.block);
+      arena.give(j.block, j.block_bytes);
       j.block = nullptr;
       return fail(VD_ERR_HIP, "job upload failed");
     }
@@ -173,10 +252,8 @@ int vd_server_create(vd_server** out, int device, const vd_config* cfg, size_t h
 void vd_server_destroy(vd_server* s) {
   if (!s) return;
   if (s->ctx) (void)vd_sync(s->ctx);
-  for (auto& kv : s->jobs)
-    if (kv.second.block) (void)hipFree(kv.second.block);
   vd_ctx_destroy(s->ctx);
-  delete s;
+  delete s;  // the arena returns its chunks
 }
 
 const char* vd_server_error(const vd_server* s) { return s ? s->err.c_str() : "null server"; }
@@ -208,6 +285,11 @@ int vd_server_submit(vd_server* s, const float* verts_xyz, int n_verts, const in
     j.box[a] = lo;
     j.box[3 + a] = hi;
   }
+  j.dev.fixed_load = vdi::fixed_point_load(verts_xyz, 0, n_verts, tets, 0, n_tets);
+  if (s->cfg.accum == VD_ACCUM_F64 && j.dev.fixed_load > vdi::kFixedPointMaxLoad)
+    return s->fail(VD_ERR_PRECONDITION,
+                   "vd_server_submit: mesh volume x largest |coordinate| exceeds the range of the exact FP64 "
+                   "fixed-point sums; map the mesh into the [0,1000]^3 frame");
   j.bytes = s->footprint(n_verts, n_tets, n_sites);
   if (j.bytes > s->budget) return s->fail(VD_ERR_NOMEM, "vd_server_submit: the job alone exceeds the memory budget");
   j.t_submit = Clock::now();
@@ -257,9 +339,11 @@ int vd_server_step(vd_server* s, int* n_running, int* n_pending) {
       const std::vector<vdi::JobDev> cur = s->devs(s->assembled);
       rc = vdi::scatter_sites(s->ctx, cur.data(), fin_idx.data(), (int)fin_idx.size());
       if (rc != VD_OK) return s->fail(rc, vd_ctx_error(s->ctx));
+      // the host waits for the device only when a job's result is due; other
+      // steps return once their iteration is enqueued
+      rc = vd_sync(s->ctx);
+      if (rc != VD_OK) return s->fail(rc, vd_ctx_error(s->ctx));
     }
-    rc = vd_sync(s->ctx);
-    if (rc != VD_OK) return s->fail(rc, vd_ctx_error(s->ctx));
     const Clock::time_point now = Clock::now();
     for (int i : fin_idx) {
       Job& j = s->jobs[s->assembled[i]];
@@ -272,7 +356,7 @@ int vd_server_step(vd_server* s, int* n_running, int* n_pending) {
         j.result[3 * k + 1] = h[k].y;
         j.result[3 * k + 2] = h[k].z;
       }
-      (void)hipFree(j.block);
+      s->arena.give(j.block, j.block_bytes);
       j.block = nullptr;
       s->used -= j.bytes;
       j.state = VD_JOB_DONE;

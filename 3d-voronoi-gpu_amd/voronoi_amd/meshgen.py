@@ -210,7 +210,7 @@ def serving_jobs(n_jobs: int, rate: float, seed: int = 21, torus_frac: float = 0
     cube of 6n^3 tets, n = round(logU[12,45]) with tets/10 sites, or (with
     probability torus_frac) a voxelised torus of 32..64 voxels per side with
     tets/10 Gaussian-mixture sites; 5..50 Lloyd iterations.  Returns a list of
-    (arrival_s, (verts, tets, sites), n_iter)."""
+    (arrival_s, (verts, tets, sites), n_iter, kind) with kind "cube" or "torus"."""
     u = uniform24(seed, 4 * n_jobs).astype(np.float64).reshape(n_jobs, 4)
     sizes = batch_sizes(n_jobs, seed=seed + 1)
     t = np.cumsum(-np.log1p(-u[:, 0]) / rate)
@@ -220,10 +220,12 @@ def serving_jobs(n_jobs: int, rate: float, seed: int = 21, torus_frac: float = 0
             nv = 32 + int(u[i, 2] * 33)
             verts, tets, _ = torus_voxels(nv)
             prob = make_problem("torus", nv, max(100, len(tets) // 10), seed=seed * 7919 + i)
+            kind = "torus"
         else:
             n = int(sizes[i])
             prob = make_problem("cube", n, max(100, (6 * n ** 3) // 10), seed=seed * 7919 + i)
-        out.append((float(t[i]), prob, 5 + int(u[i, 3] * 46)))
+            kind = "cube"
+        out.append((float(t[i]), prob, 5 + int(u[i, 3] * 46), kind))
     return out
 
 

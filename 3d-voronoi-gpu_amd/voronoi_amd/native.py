@@ -42,6 +42,7 @@ class Config(ctypes.Structure):
         ("filter", ctypes.c_int),
         ("site_records", ctypes.c_int),
         ("site_depth", ctypes.c_int),
+        ("knn_waves", ctypes.c_int),
     ]
 
 
@@ -83,14 +84,18 @@ def lib():
         L.vd_lloyd.argtypes = [P, I]
         L.vd_lloyd_graph.argtypes = [P, I]
         L.vd_autotune.argtypes = [P, I, P]
+        L.vd_autotune_k.argtypes = [P, I, ctypes.c_double, P, P]
         L.vd_get_timing.argtypes = [P, P]
         L.vd_timing_reset.argtypes = [P]
         L.vd_get_timing_sum.argtypes = [P, P, P]
+        L.vd_get_timing_list.argtypes = [P, P, I]
         L.vd_get_dims.argtypes = [P, P, P, P, P]
         L.vd_get_site_knn.argtypes = [P, P]
         L.vd_get_tet_knn.argtypes = [P, P]
         L.vd_get_moments.argtypes = [P, P]
         L.vd_get_pairs.argtypes = [P, P, P]
+        L.vd_get_pairs_range.argtypes = [P, ctypes.c_longlong, ctypes.c_longlong, P, P]
+        L.vd_get_tet_knn_range.argtypes = [P, I, I, P]
         L.vd_get_status_hist.argtypes = [P, P]
         L.vd_knn_dev.argtypes = [P, I, ctypes.c_size_t, P, I, ctypes.c_size_t, I, I, P, ctypes.c_size_t, P]
         L.vd_kn_prepare.argtypes = [P, P, I, I]
@@ -220,7 +225,16 @@ class Engine:
         """Times candidate launch parameters and keeps the fastest; returns them."""
         c = Config()
         self._ck(lib().vd_autotune(self.h, iters, ctypes.byref(c)), "vd_autotune")
+        self.cfg = c
         return {n: getattr(c, n) for n, _ in Config._fields_}
+
+    def autotune_k(self, iters=2, tol=1e-5):
+        """Smaller site_k / tet_k where the sites after `iters` iterations move
+        by at most tol * |site|; returns (chosen config, error)."""
+        c, err = Config(), ctypes.c_double()
+        self._ck(lib().vd_autotune_k(self.h, iters, tol, ctypes.byref(c), ctypes.byref(err)), "vd_autotune_k")
+        self.cfg = c
+        return {n: getattr(c, n) for n, _ in Config._fields_}, err.value
 
     def lloyd_graph(self, n_iter=1):
         """n_iter iterations replayed from one captured HIP graph."""
@@ -244,16 +258,27 @@ class Engine:
         d["iters"] = n.value
         return d
 
+    def timing_list(self, n):
+        """Phase times of each of the last n completed iterations, oldest first."""
+        t = (Timing * max(n, 1))()
+        self._ck(lib().vd_get_timing_list(self.h, t, n), "vd_get_timing_list")
+        return [{k: getattr(t[i], k) for k, _ in Timing._fields_} for i in range(n)]
+
     def site_knn(self):
         k, _, ns, _ = self.dims()
         out = np.empty((ns, k), dtype=np.int32)
         self._ck(lib().vd_get_site_knn(self.h, _p(out)), "vd_get_site_knn")
         return out
 
-    def tet_knn(self):
+    def tet_knn(self, first=None, count=None):
+        """Candidate rows of all tets, or of tets first .. first+count-1."""
         _, tk, _, nt = self.dims()
-        out = np.empty((nt, tk), dtype=np.int32)
-        self._ck(lib().vd_get_tet_knn(self.h, _p(out)), "vd_get_tet_knn")
+        if first is None:
+            out = np.empty((nt, tk), dtype=np.int32)
+            self._ck(lib().vd_get_tet_knn(self.h, _p(out)), "vd_get_tet_knn")
+            return out
+        out = np.empty((count, tk), dtype=np.int32)
+        self._ck(lib().vd_get_tet_knn_range(self.h, first, count, _p(out)), "vd_get_tet_knn_range")
         return out
 
     def moments(self):
@@ -262,12 +287,17 @@ class Engine:
         self._ck(lib().vd_get_moments(self.h, _p(out)), "vd_get_moments")
         return out
 
-    def pairs(self):
+    def pairs(self, first=None, count=None, vals=True):
+        """Per-pair status and (vol, mx, my, mz) of all pairs, or of pairs
+        first .. first+count-1 (pair = tet * TK + rank); vals=False reads the
+        statuses only."""
         _, tk, _, nt = self.dims()
-        st = np.empty(nt * tk, dtype=np.uint8)
-        vals = np.empty((nt * tk, 4), dtype=np.float64)
-        self._ck(lib().vd_get_pairs(self.h, _p(st), _p(vals)), "vd_get_pairs")
-        return st, vals
+        if first is None:
+            first, count = 0, nt * tk
+        st = np.empty(count, dtype=np.uint8)
+        v = np.empty((count, 4), dtype=np.float64) if vals else None
+        self._ck(lib().vd_get_pairs_range(self.h, first, count, _p(st), _p(v)), "vd_get_pairs_range")
+        return (st, v) if vals else st
 
     def status_hist(self):
         h = np.zeros(7, dtype=np.uint64)

@@ -15,8 +15,12 @@
 #include <hip/hip_runtime_api.h>
 #include <hip/hip_vector_types.h>
 
+#include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
+#include <map>
+#include <vector>
 
 #include "params.h"
 #include "vd_api.h"
@@ -83,8 +87,25 @@ inline void kn_free(kn_problem** kn) {
   *kn = nullptr;
 }
 
+/* Points per cell: min, max, average, total, then the histogram
+ * (knearests.cu:434-460), on stderr. */
 inline void kn_print_stats(kn_problem* kn) {
-  fprintf(stderr, "Grid: %d x %d x %d cells, %d points\n", kn->dimx, kn->dimy, kn->dimz, kn->allocated_points);
+  const size_t cells = (size_t)kn->dimx * kn->dimy * kn->dimz;
+  std::vector<int> counters(cells);
+  if (vd_kn_get_cell_counts(reinterpret_cast<vd_compat::KnHolder*>(kn)->impl, counters.data()) != VD_OK)
+    vd_compat::kn_die("kn_print_stats");
+  long long tot = 0;
+  int cmin = INT_MAX, cmax = 0;
+  std::map<int, int> histo;
+  for (int c : counters) {
+    histo[c]++;
+    cmin = std::min(cmin, c);
+    cmax = std::max(cmax, c);
+    tot += c;
+  }
+  fprintf(stderr, "Grid:  points per cell: %d (min), %d (max), %g avg, total %lld\n", cmin, cmax,
+          kn->allocated_points / (double)cells, tot);
+  for (const auto& h : histo) fprintf(stderr, "[%d] => %d\n", h.first, h.second);
 }
 
 /* Host copies; the caller free()s them (knearests.cu:404-432). */

@@ -97,6 +97,8 @@ typedef struct vd_config {
   int site_depth;     /* site-major form: bisectors per site table of the launch; a pair that
                          needs more is recomputed from scratch at site_k by the retry pass,
                          so results do not depend on it; 0 = automatic (40) */
+  int knn_waves;      /* wavefronts per workgroup of the kNN queries: 2, 4 or 8; 0 = automatic
+                         (4); results do not depend on it */
 } vd_config;
 
 typedef struct vd_timing {
@@ -115,7 +117,7 @@ typedef struct vd_ctx vd_ctx;
 /* Defaults: site_k 90, tet_k 0, pcap 64, tcap 96, VD_ACCUM_F32,
  * VD_MOMENTS_REFERENCE, VD_FORM_SITE_MAJOR, grid_density 3.1
  * (knearests.cu:243), record_pairs 0, clip_blocks 0, site_tcap 0, cull_depth 16, srnr_k 0,
- * filter 0, site_records 0, site_depth 0. */
+ * filter 0, site_records 0, site_depth 0, knn_waves 0. */
 void vd_config_default(vd_config* cfg);
 
 /* Context on HIP device `device`.  Replaces the device selection and buffer
@@ -167,12 +169,25 @@ int vd_lloyd(vd_ctx* ctx, int n_iter);
 int vd_lloyd_graph(vd_ctx* ctx, int n_iter);
 /* Chooses the launch parameters for the loaded problem: times `iters`
  * iterations (after one untimed) for candidate values of site_tcap,
- * cull_depth and grid_density, one parameter at a time, keeps the fastest
- * configuration (written to *chosen when given) and restores the sites.
- * These parameters change no per-pair result, so in the VD_ACCUM_F64 mode the
- * results are bit for bit those of any other choice.  No counterpart in the
- * reference (its launch shapes are constants, voronoi.cu:1034-1076). */
+ * cull_depth, grid_density and the block sizes (clip_blocks, the site-major
+ * launch's workgroups; knn_waves, the kNN queries' waves per workgroup), one
+ * parameter at a time, keeps the fastest configuration (written to *chosen
+ * when given) and restores the sites.  These parameters change no per-pair
+ * result, so in the VD_ACCUM_F64 mode the results are bit for bit those of
+ * any other choice.  No counterpart in the reference (its launch shapes are
+ * constants, voronoi.cu:1034-1076). */
 int vd_autotune(vd_ctx* ctx, int iters, vd_config* chosen);
+/* Chooses smaller neighbour and candidate counts than the configured site_k
+ * and tet_k where they change the result by at most `tol`: from the loaded
+ * sites it runs `iters` iterations with the configured counts, then with each
+ * candidate tet_k (from 0.875 down to 0.3 of the configured one, at least 8)
+ * and site_k (the same fractions, at least 16), and keeps the fastest
+ * configuration whose sites after `iters` iterations differ from the
+ * configured ones by at most tol * |site| (largest relative site displacement;
+ * 0 demands bit-identical sites).  That error is written to *err, the
+ * configuration to *chosen and into the context; the sites are restored.
+ * The reference fixes these counts (params.h:4-25, voronoi.cu:936). */
+int vd_autotune_k(vd_ctx* ctx, int iters, double tol, vd_config* chosen, double* err);
 int vd_sync(vd_ctx* ctx);
 /* Device times (HIP events on the context's stream) of the phases of the
  * iteration in progress, or of the last completed one. */
@@ -181,6 +196,11 @@ int vd_get_timing(vd_ctx* ctx, vd_timing* t);
  * (at most the last 256), and their number. */
 int vd_timing_reset(vd_ctx* ctx);
 int vd_get_timing_sum(vd_ctx* ctx, vd_timing* t, int* n_iters);
+/* Phase times of each of the last n completed iterations (n at most the number
+ * completed since vd_timing_reset, and at most 256), oldest first.  The
+ * reference appends the same per-iteration times to record.csv
+ * (voronoi.cu:984-1175). */
+int vd_get_timing_list(vd_ctx* ctx, vd_timing* t, int n);
 
 /* Effective sizes: K = min(site_k, n_sites-1), TK = min(tet_k or heuristic, n_sites). */
 int vd_get_dims(vd_ctx* ctx, int* k, int* tk, int* n_sites, int* n_tets);
@@ -193,6 +213,12 @@ int vd_get_moments(vd_ctx* ctx, double* out);
 /* Per-pair status and (vol, mx, my, mz) of the last clip step; needs
  * record_pairs.  Pair index = tet * TK + rank. */
 int vd_get_pairs(vd_ctx* ctx, uint8_t* status, double* vals);
+/* Pairs first .. first+count-1 of vd_get_pairs (either output may be NULL),
+ * for results too large to read whole.  No counterpart in the reference, which
+ * keeps per-pair statuses on the device (voronoi.cu:942,979). */
+int vd_get_pairs_range(vd_ctx* ctx, long long first, long long count, uint8_t* status, double* vals);
+/* Rows first_tet .. first_tet+n_tets-1 of vd_get_tet_knn. */
+int vd_get_tet_knn_range(vd_ctx* ctx, int first_tet, int n_tets, int32_t* out);
 /* Number of pairs per status value (7 counters) of the last clip step. */
 int vd_get_status_hist(vd_ctx* ctx, uint64_t* hist7);
 
@@ -230,6 +256,9 @@ int vd_kn_info_get(const vd_kn* kn, vd_kn_info* info);
 int vd_kn_get_points(vd_kn* kn, float* out_xyz);
 int vd_kn_get_knearests(vd_kn* kn, unsigned int* out);
 int vd_kn_get_permutation(vd_kn* kn, unsigned int* out);
+/* Points per grid cell, dimx * dimy * dimz counts (x fastest), the data of
+ * kn_print_stats (knearests.cu:434-460). */
+int vd_kn_get_cell_counts(vd_kn* kn, int* out);
 
 /* ---- job server: continuous batching of independent jobs ---------------- */
 

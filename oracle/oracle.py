@@ -65,7 +65,14 @@ def _p(a):
 
 
 def nthreads() -> int:
-    return int(os.environ.get("VD_ORACLE_THREADS", os.cpu_count() or 1))
+    """VD_ORACLE_THREADS, else OMP_NUM_THREADS, else the CPUs this process may run on."""
+    for var in ("VD_ORACLE_THREADS", "OMP_NUM_THREADS"):
+        if os.environ.get(var, "").strip().isdigit() and int(os.environ[var]) > 0:
+            return int(os.environ[var])
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
 def knn(ref, qry, k, self_exclude, grid=True, threads=None):

@@ -122,6 +122,54 @@ def test_autotune_keeps_results(vdn):
         e.set_sites(s)
         chosen = e.autotune(iters=1)
         assert chosen["site_tcap"] in (0, 12, 16, 20, 24) and chosen["cull_depth"] in (8, 16, 32, 64)
+        assert chosen["clip_blocks"] in (0, 1536, 3072, 4608) and chosen["knn_waves"] in (0, 2, 4, 8)
         np.testing.assert_array_equal(e.sites(), s.astype(np.float32))  # restored
         e.lloyd(2)
         np.testing.assert_array_equal(e.sites(), want)
+
+
+@pytest.mark.parametrize("knn_waves,clip_blocks", [(2, 1536), (8, 4608)])
+def test_block_sizes_keep_results(vdn, knn_waves, clip_blocks):
+    """The kNN workgroup size and the site-major launch's workgroup count
+    change no result (FP64 accumulation, two iterations)."""
+    from voronoi_amd import meshgen
+
+    v, t, s = meshgen.make_problem("cube", 12, 1500, seed=5)
+    out = []
+    for cfg in (dict(), dict(knn_waves=knn_waves, clip_blocks=clip_blocks)):
+        with vdn.Engine(accum="f64", tet_k=32, **cfg) as e:
+            e.set_mesh(v, t)
+            e.set_sites(s)
+            e.lloyd(2)
+            out.append((e.sites(), e.site_knn(), e.tet_knn()))
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("tol", [0.0, 1e-5])
+def test_autotune_k(vdn, tol):
+    """vd_autotune_k keeps only smaller neighbour/candidate counts whose sites
+    after `iters` iterations stay within tol * |site| of the configured
+    counts' (tol 0: the same bits); the reported error is checked here by
+    running both configurations."""
+    from voronoi_amd import meshgen
+
+    v, t, s = meshgen.make_problem("cube", 12, 1500, seed=5)
+    base = dict(accum="f64", tet_k=64, site_k=90)
+    with vdn.Engine(**base) as e:
+        e.set_mesh(v, t)
+        e.set_sites(s)
+        e.lloyd(2)
+        want = e.sites()
+    with vdn.Engine(**base) as e:
+        e.set_mesh(v, t)
+        e.set_sites(s)
+        chosen, err = e.autotune_k(iters=2, tol=tol)
+        assert chosen["tet_k"] <= 64 and chosen["site_k"] <= 90 and 0.0 <= err <= tol
+        np.testing.assert_array_equal(e.sites(), s.astype(np.float32))  # restored
+        e.lloyd(2)
+        got = e.sites()
+    d = np.linalg.norm(got.astype(np.float64) - want, axis=1) / np.linalg.norm(want.astype(np.float64), axis=1)
+    assert d.max() <= tol
+    assert abs(d.max() - err) <= 1e-12
+    print(f"tol {tol}: tet_k {chosen['tet_k']} site_k {chosen['site_k']} err {err:.3g}")

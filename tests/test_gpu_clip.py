@@ -376,3 +376,29 @@ def test_arithmetic_filter_bit_exact(vdn, O, case, form, filt):
     np.testing.assert_array_equal(g["status"].astype(np.int32), r["pair_status"])
     np.testing.assert_array_equal(g["vals"], r["pair_vals"])
     np.testing.assert_array_equal(g["moments"], r["site_vals"])
+
+
+def test_fixed_point_range_guard(vdn):
+    """The exact FP64 site sums (64-bit integers in units of 2^-20) hold while
+    mesh volume x largest |coordinate| <= 2^41: a mesh ten times the
+    [0,1000]^3 frame (1e12 x 1e4) is refused before clipping in that mode, and
+    runs in the FP32 mode; the frame itself (<= 1e12) runs."""
+    verts, tets, sites = problem("cube", 6, 200)
+    big_v, big_s = verts * np.float32(10), sites * np.float32(10)
+    with vdn.Engine(accum="f64") as e:
+        e.set_mesh(big_v, tets)
+        e.set_sites(big_s)
+        e.knn_step()
+        with pytest.raises(RuntimeError, match="fixed-point"):
+            e.clip_step()
+        e.set_mesh(verts, tets)
+        e.set_sites(sites)
+        e.lloyd(1)
+    with vdn.Engine(accum="f32") as e:
+        e.set_mesh(big_v, tets)
+        e.set_sites(big_s)
+        e.lloyd(1)
+        assert np.all(np.isfinite(e.sites()))
+    with vdn.Server(accum="f64", tet_k=32) as s:
+        with pytest.raises(RuntimeError, match="fixed-point"):
+            s.submit(big_v, tets, big_s, 1)

@@ -140,3 +140,42 @@ def test_kn_problem_sorted_ids(vdn, O):
     info = kn.info()
     assert info.allocated_points == len(sites) and info.dimx >= 1
     kn.close()
+
+
+@pytest.mark.parametrize("kind,n,ns,tk", [("cube", 12, 3000, 64), ("torus", 20, 2500, 96)])
+def test_knn_bounded_by_previous_rows(vdn, O, kind, n, ns, tk):
+    """From the second kNN step on, each query starts from the k points of its
+    previous row; the rows must still equal the oracle's exact kNN, after Lloyd
+    moves, after the sites are replaced by an unrelated set of the same size
+    (the old rows then bound loosely), and in a batch."""
+    from voronoi_amd import meshgen
+
+    verts, tets, sites = problem(kind, n, ns)
+    other = meshgen.make_problem(kind, n, ns, seed=99)[2]
+    with _engine(vdn, verts, tets, sites, tet_k=tk, accum="f64") as e:
+        for step in range(4):
+            if step == 3:
+                e.set_sites(other)
+            cur = e.sites()
+            e.knn_step()
+            print(f"{kind} step {step}", flush=True)
+            np.testing.assert_array_equal(e.site_knn(), O.site_knn(cur, 90))
+            np.testing.assert_array_equal(e.tet_knn(), O.tet_knn(verts, tets, cur, tk))
+            e.clip_step()
+            e.update_step()
+    probs = [meshgen.make_problem("cube", 7, 400, seed=5), meshgen.make_problem(kind, n, ns, seed=6)]
+    with vdn.Engine(tet_k=tk, accum="f64") as e:
+        e.set_batch(probs)
+        off = e.batch_sites
+        for step in range(3):
+            cur = e.sites()
+            e.knn_step()
+            sk, tkn = e.site_knn(), e.tet_knn()
+            t0 = 0
+            for i, (v, t, _) in enumerate(probs):
+                s = cur[off[i]:off[i + 1]]
+                np.testing.assert_array_equal(sk[off[i]:off[i + 1]] - off[i], O.site_knn(s, e.dims()[0]))
+                np.testing.assert_array_equal(tkn[t0:t0 + len(t)] - off[i], O.tet_knn(v, t, s, tk))
+                t0 += len(t)
+            e.clip_step()
+            e.update_step()
