@@ -1,5 +1,6 @@
 // C ABI entry points that stand in for the reference's own host functions:
 //   vd_knn_dev                       <- knn_cuda_global_dev (knncuda.cu:271-349)
+//   vd_knn_dev_noalloc               the same kNN on a static device scratch array
 //   vd_kn_prepare / solve / free ... <- kn_prepare, kn_solve, kn_free, kn_get_*
 //                                       (knearests.cu:230-432)
 //   vd_compute_clipped_voro_diagram  <- compute_clipped_voro_diagram_GPU
@@ -11,6 +12,8 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -90,7 +93,7 @@ struct GridTmp {
   }
 };
 
-// Arguments of vd_knn_dev (knncuda.cu:271-349: pitched
+// Arguments of vd_knn_dev and vd_knn_dev_noalloc (knncuda.cu:271-349: pitched
 // SoA rows, dim 3; the index pitch must hold every query).
 bool knn_dev_args_ok(const float* ref_dev, int ref_nb, size_t ref_pitch, const float* query_dev, int query_nb,
                      size_t query_pitch, int dim, int k, const int* knn_index_dev, size_t index_pitch) {
@@ -114,6 +117,96 @@ hipError_t knn_dev_enqueue(const vd::GridBufs& gb, const int4* offsets, int n_of
     e = vd::knn_query(gb, ref_nb, offsets, n_offsets, k, 2, query_nb, nullptr, nullptr, q4, nullptr, knn_index_dev,
                       (long long)index_pitch, st);
   return e;
+}
+
+// Device scratch of vd_knn_dev_noalloc, which allocates nothing: one array per
+// device, held until the process ends.  Host code never uses the array's name
+// as an address: each device's address comes from hipGetSymbolAddress at the
+// first call on that device (noalloc_slot).  Layout (scratch_layout): the cell
+// offset table, the grid shape, the grid's per-cell and per-point tables for
+// up to VD_KNN_NOALLOC_MAX_REF reference points, the reference points and the
+// queries as float4.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+struct ScratchLayout {
+  vd::GridBufs grid;
+  int4* offsets = nullptr;
+  float4* ref4 = nullptr;
+  float4* q4 = nullptr;
+  size_t bytes = 0;
+};
+
+// Carves the scratch at `base` (a device address); the grid's cell count is
+// that of n_ref points, its buffers are sized for the maximum.
+ScratchLayout scratch_layout(unsigned char* base, int n_ref) {
+  ScratchLayout L;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base + off;
+    off += (bytes + 255) & ~(size_t)255;
+    return p;
+  };
+  const int max_ref = VD_KNN_NOALLOC_MAX_REF;
+  const int max_cells = vd::grid_max_cells(max_ref, 3.1f);
+  const int n_offsets = (2 * vd::kOffsetRadius + 1) * (2 * vd::kOffsetRadius + 1) * (2 * vd::kOffsetRadius + 1);
+  L.offsets = (int4*)take(sizeof(int4) * n_offsets);
+  L.grid.shape = (vd::GridShape*)take(sizeof(vd::GridShape));
+  L.grid.count = (int*)take(sizeof(int) * max_cells);
+  L.grid.start = (int*)take(sizeof(int) * ((size_t)max_cells + 1));
+  L.grid.cursor = (int*)take(sizeof(int) * max_cells);
+  L.grid.scan_tmp = (int*)take(sizeof(int) * vd::scan_tmp_ints(max_cells));
+  L.grid.cell_of = (int*)take(sizeof(int) * max_ref);
+  L.grid.sorted_ids = (int*)take(sizeof(int) * max_ref);
+  L.grid.sorted_pts = (float4*)take(sizeof(float4) * max_ref);
+  L.ref4 = (float4*)take(sizeof(float4) * max_ref);
+  L.q4 = (float4*)take(sizeof(float4) * (size_t)VD_KNN_NOALLOC_MAX_QUERY);
+  L.grid.max_cells = vd::grid_max_cells(std::min(n_ref, max_ref), 3.1f);
+  L.bytes = off;
+  return L;
+}
+
+struct NoAllocSlot {
+  unsigned char* ptr = nullptr;  // this device's address of `scratch`
+  int n_offsets = 0;
+};
+std::mutex g_noalloc_mu;
+std::map<int, NoAllocSlot> g_noalloc;
+
+// Set-up of the current device `dev` at its first call: the scratch's
+// address, and the cell offset table uploaded into it (a synchronous copy of
+// a constant table; later calls only enqueue kernels).
+int noalloc_slot(int dev, const NoAllocSlot** out) {
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  auto it = g_noalloc.find(dev);
+  if (it != g_noalloc.end()) {
+    *out = &it->second;
+    return VD_OK;
+  }
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    fprintf(stderr, "vd_knn_dev_noalloc: no address for the scratch on device %d: %s\n", dev, hipGetErrorString(e));
+    return VD_ERR_HIP;
+  }
+  NoAllocSlot slot;
+  slot.ptr = (unsigned char*)p;
+  const ScratchLayout L = scratch_layout(slot.ptr, VD_KNN_NOALLOC_MAX_REF);
+  const std::vector<int4> t = vd::knn_offset_table();
+  if (L.bytes > kScratchBytes || (size_t)((unsigned char*)L.grid.shape - (unsigned char*)L.offsets) <
+                                     sizeof(int4) * t.size()) {
+    fprintf(stderr, "vd_knn_dev_noalloc: scratch layout of %zu bytes does not fit %zu\n", L.bytes, kScratchBytes);
+    return VD_ERR_HIP;
+  }
+  e = hipMemcpy(L.offsets, t.data(), sizeof(int4) * t.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    fprintf(stderr, "vd_knn_dev_noalloc: cannot set up device %d: %s\n", dev, hipGetErrorString(e));
+    return VD_ERR_HIP;
+  }
+  slot.n_offsets = (int)t.size();
+  auto r = g_noalloc.emplace(dev, slot);
+  *out = &r.first->second;
+  return VD_OK;
 }
 
 }  // namespace
@@ -147,6 +240,30 @@ int vd_knn_dev(const float* ref_dev, int ref_nb, size_t ref_pitch, const float* 
                       query_nb, query_pitch, k, knn_index_dev, index_pitch, st))
     return VD_ERR_HIP;
   if ((e = hipStreamSynchronize(st))) return VD_ERR_HIP;
+  return VD_OK;
+}
+
+int vd_knn_dev_noalloc(const float* ref_dev, int ref_nb, size_t ref_pitch, const float* query_dev, int query_nb,
+                       size_t query_pitch, int dim, int k, int* knn_index_dev, size_t index_pitch, void* stream) {
+  if (!knn_dev_args_ok(ref_dev, ref_nb, ref_pitch, query_dev, query_nb, query_pitch, dim, k, knn_index_dev,
+                       index_pitch))
+    return VD_ERR_ARG;
+  if (ref_nb > VD_KNN_NOALLOC_MAX_REF || query_nb > VD_KNN_NOALLOC_MAX_QUERY) {
+    fprintf(stderr,
+            "vd_knn_dev_noalloc: %d reference points and %d queries exceed its scratch limits "
+            "(%d and %d); use vd_knn_dev\n",
+            ref_nb, query_nb, VD_KNN_NOALLOC_MAX_REF, VD_KNN_NOALLOC_MAX_QUERY);
+    return VD_ERR_ARG;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return VD_ERR_HIP;
+  const NoAllocSlot* slot = nullptr;
+  const int rc = noalloc_slot(dev, &slot);
+  if (rc != VD_OK) return rc;
+  const ScratchLayout L = scratch_layout(slot->ptr, ref_nb);
+  if (knn_dev_enqueue(L.grid, L.offsets, slot->n_offsets, L.ref4, L.q4, ref_dev, ref_nb, ref_pitch, query_dev,
+                      query_nb, query_pitch, k, knn_index_dev, index_pitch, (hipStream_t)stream))
+    return VD_ERR_HIP;
   return VD_OK;
 }
 

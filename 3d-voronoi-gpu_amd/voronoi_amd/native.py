@@ -98,6 +98,7 @@ def lib():
         L.vd_get_tet_knn_range.argtypes = [P, I, I, P]
         L.vd_get_status_hist.argtypes = [P, P]
         L.vd_knn_dev.argtypes = [P, I, ctypes.c_size_t, P, I, ctypes.c_size_t, I, I, P, ctypes.c_size_t, P]
+        L.vd_knn_dev_noalloc.argtypes = L.vd_knn_dev.argtypes
         L.vd_kn_prepare.argtypes = [P, P, I, I]
         L.vd_kn_solve.argtypes = [P]
         L.vd_kn_free.argtypes = [P]

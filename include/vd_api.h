@@ -233,6 +233,24 @@ int vd_get_status_hist(vd_ctx* ctx, uint64_t* hist7);
 int vd_knn_dev(const float* ref_dev, int ref_nb, size_t ref_pitch, const float* query_dev, int query_nb,
                size_t query_pitch, int dim, int k, int* knn_index_dev, size_t index_pitch, void* stream);
 
+/* The kNN of vd_knn_dev for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments and results as vd_knn_dev,
+ * but it allocates nothing: its working memory (grid tables, points, queries)
+ * is a static device array of 32 MiB per GPU, held until the process ends,
+ * whose address is looked up once per device at the first call on it.  That
+ * first call on a device also uploads a constant table synchronously, so make
+ * it outside any graph capture; every later call only enqueues kernels on
+ * `stream` and returns without synchronising.  At most VD_KNN_NOALLOC_MAX_REF
+ * reference points and VD_KNN_NOALLOC_MAX_QUERY queries (VD_ERR_ARG beyond,
+ * with a message on stderr).  All calls on one GPU share its array: calls that
+ * overlap on one GPU (different streams or host threads without ordering)
+ * race and give wrong results. */
+#define VD_KNN_NOALLOC_MAX_REF 262144
+#define VD_KNN_NOALLOC_MAX_QUERY 1048576
+int vd_knn_dev_noalloc(const float* ref_dev, int ref_nb, size_t ref_pitch, const float* query_dev, int query_nb,
+                       size_t query_pitch, int dim, int k, int* knn_index_dev, size_t index_pitch, void* stream);
+
 /* Grid kNN problem, replacing kn_prepare / kn_solve / kn_free and the getters
  * (knearests.h:3-27, knearests.cu:230-432).  Points are host xyz triples; the
  * device copy is reordered by grid cell (deterministically: by cell, then by
