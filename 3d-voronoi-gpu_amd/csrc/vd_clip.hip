@@ -641,38 +641,81 @@ struct Cell {
     }
   }
 
-  // Same ring around the mean of the vertices, with edge feet taken on the
-  // edge line (the library's well-conditioned formulation).
+  // FP64 plane of plane id `id` rebuilt from the FP32 inputs (vd_geom.hpp):
+  // cell_nv < 0, a tet -> cell polytope: ids 0-3 the tet's faces, later ids
+  // bisectors; cell_nv >= 0, a cell clipped by a tet (kernel C): ids 0-5 the
+  // box (exact in FP32), ids below cell_nv bisectors, later ids the tet's
+  // faces, each recognised by its FP32 plane.
+  __device__ D4 plane64(int id, const P4 v[4], int cell_nv) const {
+    const P4 p = ps.get(id);
+    int f = id;
+    if (cell_nv < 0) {
+      if (id >= 4) return bisector64(p, seed);
+    } else {
+      if (id < 6) return up4(p);
+      if (id < cell_nv) return bisector64(p, seed);
+      f = -1;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const P4 q = face_plane(k, v);
+        if (__float_as_uint(q.x) == __float_as_uint(p.x) && __float_as_uint(q.y) == __float_as_uint(p.y) &&
+            __float_as_uint(q.z) == __float_as_uint(p.z) && __float_as_uint(q.w) == __float_as_uint(p.w) && f < 0)
+          f = k;
+      }
+      if (f < 0) return up4(p);
+    }
+    if (f == 0) return face64(v[2], v[1], v[3]);
+    if (f == 1) return face64(v[0], v[2], v[3]);
+    if (f == 2) return face64(v[1], v[0], v[3]);
+    return face64(v[0], v[1], v[2]);
+  }
+  // FP32 plane of face k of the tet, as put_tet_faces stores it (k constant)
+  __device__ __forceinline__ static P4 face_plane(int k, const P4 v[4]) {
+    return k == 0 ? tri2plane(v[2], v[1], v[3])
+                  : k == 1 ? tri2plane(v[0], v[2], v[3]) : k == 2 ? tri2plane(v[1], v[0], v[3]) : tri2plane(v[0], v[1], v[2]);
+  }
+
+  // The same six-tet ring around an apex O, with edge feet taken on the edge
+  // line (the library's well-conditioned formulation), integrated in FP64 over
+  // planes rebuilt in FP64; v: the tet's corners.  The signed ring is exact for
+  // any apex, so O is the mean of the FP32 vertices.
   template <int ACC>
-  __device__ void moments_robust(PairSums<ACC>& S) const {
-    P4 O = {0.f, 0.f, 0.f, 1.f};
+  __device__ void moments_robust(PairSums<ACC>& S, const P4 v[4], int cell_nv) const {
+    P4 Of = {0.f, 0.f, 0.f, 1.f};
     for (int t = 0; t < nt; ++t) {
       const P4 X = vertex(tri(t));
-      O.x += X.x;
-      O.y += X.y;
-      O.z += X.z;
+      Of.x += X.x;
+      Of.y += X.y;
+      Of.z += X.z;
     }
     const float inv = 1.f / (float)(nt > 0 ? nt : 1);
-    O.x *= inv;
-    O.y *= inv;
-    O.z *= inv;
+    Of.x *= inv;
+    Of.y *= inv;
+    Of.z *= inv;
+    const D4 O = up4(Of);
+    double sv = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;  // 6 x volume, 24 x moments
     for (int t = 0; t < nt; ++t) {
       const uint32_t tt = tri(t);
-      const P4 q[3] = {ps.get(tri_a(tt)), ps.get(tri_b(tt)), ps.get(tri_c(tt))};
-      const P4 X = vertex3(q[0], q[1], q[2]);
-      P4 P[6];
+      const D4 q[3] = {plane64(tri_a(tt), v, cell_nv), plane64(tri_b(tt), v, cell_nv),
+                       plane64(tri_c(tt), v, cell_nv)};
+      const D4 X = vertex3d(q[0], q[1], q[2]);
+      D4 P[6];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) P[2 * k] = foot_on_plane(O, q[k]);
+      for (int k = 0; k < 3; ++k) P[2 * k] = foot_on_plane_d(O, q[k]);
 #pragma unroll
-      for (int k = 0; k < 3; ++k) P[2 * k + 1] = foot_on_line(O, X, cross3(q[k], q[(k + 1) % 3]));
+      for (int k = 0; k < 3; ++k) P[2 * k + 1] = foot_on_line_d(O, X, cross3d(q[k], q[(k + 1) % 3]));
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
-        float v;
-        P4 b;
-        tet_vol_bary(P[k], P[(k + 1) % 6], O, X, v, b);
-        S.add(v, b);
+        double vol6;
+        D4 b4;
+        tet_vol6_bary4_d(P[k], P[(k + 1) % 6], O, X, vol6, b4);
+        sx = fma(vol6, b4.x, sx);
+        sy = fma(vol6, b4.y, sy);
+        sz = fma(vol6, b4.z, sz);
+        sv += vol6;
       }
     }
+    S.set(sv / 6.0, sx / 24.0, sy / 24.0, sz / 24.0);
   }
 };
 
@@ -709,12 +752,25 @@ __device__ __forceinline__ void count_status(const ClipArgs& a, int status, bool
 
 __device__ __forceinline__ P4 site_at(const ClipArgs& a, int i) { return ld4(a.sites, i); }
 
+// Corners of a tet from its record (x0 y0 z0 x1 | y1 z1 x2 y2 | z2 x3 y3 z3).
+__device__ __forceinline__ void corners_of(const float4& q0, const float4& q1, const float4& q2, P4 v[4]) {
+  v[0] = {q0.x, q0.y, q0.z, 0.f};
+  v[1] = {q0.w, q1.x, q1.y, 0.f};
+  v[2] = {q1.z, q1.w, q2.x, 0.f};
+  v[3] = {q2.y, q2.z, q2.w, 0.f};
+}
+__device__ __forceinline__ void tet_corners(const ClipArgs& a, int t, P4 v[4]) {
+  const float4* r = a.tet_xyz + 3 * (size_t)t;
+  corners_of(r[0], r[1], r[2], v);
+}
+
+
 // Clips one pair whose cell has been set up; returns through S and status.
 // kernel A's loop (voronoi.cu:679-700) with the bisector of the v-th neighbour
 // under plane id pid(v).
 template <int MF, int ACC, class PS, class TS, class PlaneOf>
 __device__ __forceinline__ bool run_pair(const ClipArgs& a, Cell<PS, TS>& c, const int32_t* nbr, PlaneOf plane_of,
-                                         PairSums<ACC>& S) {
+                                         int tet, PairSums<ACC>& S) {
   const int K = a.K;
   bool stopped = false;
   if constexpr (TS::kCached) {
@@ -798,10 +854,13 @@ __device__ __forceinline__ bool run_pair(const ClipArgs& a, Cell<PS, TS>& c, con
     if (!ok) c.status = kSecurityRadiusNotReached;
   }
   if (c.status == kNoIntersection) return false;
-  if (MF == 0)
+  if (MF == 0) {
     c.template moments_ref<ACC>(S);
-  else
-    c.template moments_robust<ACC>(S);
+  } else {
+    P4 v[4];
+    tet_corners(a, tet, v);
+    c.template moments_robust<ACC>(S, v, -1);
+  }
   // vertices formed by parallel planes (degenerate input) give non-finite sums;
   // such a pair is not accumulated
   if (!(isfinite((double)S.v) && isfinite((double)S.x) && isfinite((double)S.y) && isfinite((double)S.z))) {
@@ -869,7 +928,7 @@ __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
       pid = c.nv;  // next free position
       if (pid < a.pcap) c.ps.put(pid, eq);
     };
-    acc = run_pair<MF, ACC>(ak, c, nbr, plane_of, S);
+    acc = run_pair<MF, ACC>(ak, c, nbr, plane_of, t, S);
     status = c.status;
     if (a.retry_list && ((status == kTriangleOverflow && a.tcap < a.tcap_full) ||
                          (status == kVertexOverflow && a.pcap < a.pcap_full))) {
@@ -926,18 +985,6 @@ constexpr int kRecChunk = 128;           // record slots a wave reserves at a ti
 // own, so that a wave of the volume pass (64 records of one chunk) runs over
 // polytopes of similar size.
 constexpr int kRecSplit = 9;
-
-// Corners of a tet from its record (x0 y0 z0 x1 | y1 z1 x2 y2 | z2 x3 y3 z3).
-__device__ __forceinline__ void corners_of(const float4& q0, const float4& q1, const float4& q2, P4 v[4]) {
-  v[0] = {q0.x, q0.y, q0.z, 0.f};
-  v[1] = {q0.w, q1.x, q1.y, 0.f};
-  v[2] = {q1.z, q1.w, q2.x, 0.f};
-  v[3] = {q2.y, q2.z, q2.w, 0.f};
-}
-__device__ __forceinline__ void tet_corners(const ClipArgs& a, int t, P4 v[4]) {
-  const float4* r = a.tet_xyz + 3 * (size_t)t;
-  corners_of(r[0], r[1], r[2], v);
-}
 
 __host__ __device__ inline size_t site_lds_layout(int K, int tcap, size_t* off) {
   const size_t K4 = (size_t)((K + 3) & ~3);
@@ -1097,23 +1144,29 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
       lst = kIdle;
     };
 
-    // Volume and moments of this lane's polytope, in place.
+    // Volume and moments of this lane's polytope, in place, in the reference's
+    // FP32 decomposition.  The robust decomposition's FP64 moments stay out of
+    // this kernel (their registers would cost it resident waves): such a pair
+    // goes to the retry list instead, whose pass computes the same result.
     auto moments_now = [&]() {
-      PairSums<ACC> S;
-      if (MF == 0)
+      if constexpr (MF == 0) {
+        PairSums<ACC> S;
         c.template moments_ref<ACC>(S);
-      else
-        c.template moments_robust<ACC>(S);
-      bool acc = false;
-      // vertices formed by parallel planes (degenerate input) give
-      // non-finite sums; such a pair is not accumulated
-      if (!(isfinite((double)S.v) && isfinite((double)S.x) && isfinite((double)S.y) && isfinite((double)S.z)))
-        c.status = kNeedsExactPredicates;
-      else if (fabs((double)S.v) < 0.1)
-        c.status = kNoIntersection;
-      else
-        acc = true;
-      finish(S, acc);
+        bool acc = false;
+        // vertices formed by parallel planes (degenerate input) give
+        // non-finite sums; such a pair is not accumulated
+        if (!(isfinite((double)S.v) && isfinite((double)S.x) && isfinite((double)S.y) && isfinite((double)S.z)))
+          c.status = kNeedsExactPredicates;
+        else if (fabs((double)S.v) < 0.1)
+          c.status = kNoIntersection;
+        else
+          acc = true;
+        finish(S, acc);
+      } else {
+        const unsigned slot = atomicAdd(a.retry_count, 1u);
+        if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = ppair;
+        lst = kIdle;
+      }
     };
 
     bool defer = false;
@@ -1174,7 +1227,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         // place, so that the list is final when this kernel ends (the re-clip
         // that reads it runs beside the volume pass, not after it).
         if (c.status == kTriangleOverflow && a.retry_list && a.tcap < a.tcap_full)
-          moments_now();
+          moments_now();  // the robust form: retried without the volume test
         else
           defer = true;
       } else if (out != 0) {
@@ -1514,7 +1567,7 @@ __global__ __launch_bounds__(kMomentsBlock) void k_site_moments(ClipArgs a) {
       if (MF == 0)
         c.template moments_ref<ACC>(S);
       else
-        c.template moments_robust<ACC>(S);
+        c.template moments_robust<ACC>(S, v, -1);
       if (!(isfinite((double)S.v) && isfinite((double)S.x) && isfinite((double)S.y) && isfinite((double)S.z)))
         status = kNeedsExactPredicates;
       else if (fabs((double)S.v) < 0.1)
@@ -1697,7 +1750,7 @@ __global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs) {
           if (MF == 0)
             c.template moments_ref<ACC>(S);
           else
-            c.template moments_robust<ACC>(S);
+            c.template moments_robust<ACC>(S, v, cnv);  // planes below cnv: the cell's
           if (!(isfinite((double)S.v) && isfinite((double)S.x) && isfinite((double)S.y) && isfinite((double)S.z)))
             c.status = kNeedsExactPredicates;
           else if (fabs((double)S.v) < 0.1)
@@ -2032,6 +2085,9 @@ hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, i
   }
   if (!a.site_counter) return hipErrorInvalidValue;
   if (a.rec_meta && (!a.rec_tris || !a.rec_count)) return hipErrorInvalidValue;
+  // the robust moments are never computed in the site-major kernel: the
+  // polytopes it cannot hand to the volume pass go to the retry list
+  if (moments == 1 && (!a.retry_list || !a.retry_count)) return hipErrorInvalidValue;
   if (a.site_depth > 0 && a.site_depth < a.K && (!a.retry_list || !a.retry_count)) return hipErrorInvalidValue;
   const size_t lds = clip_site_lds(a.site_depth > 0 ? std::min(a.site_depth, a.K) : a.K, a.tcap);
   hipError_t e;

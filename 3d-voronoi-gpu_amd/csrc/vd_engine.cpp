@@ -771,7 +771,9 @@ int vd_clip_step(vd_ctx* ctx) {
     a.site_counter = ctx->site_counter.p;
     const int fast_tcap = vdi::site_tcap_of(c);
     a.site_depth = site_depth_of(c, ctx->K);
-    if (fast_tcap < c.tcap || a.site_depth < ctx->K) {
+    // (the robust moments also need it: the site-major kernel retries the
+    // polytopes it cannot hand to the volume pass)
+    if (fast_tcap < c.tcap || a.site_depth < ctx->K || c.moments == VD_MOMENTS_ROBUST) {
       VD_HIP(ctx->retry_list.ensure(std::max(n_pairs, 1LL)), "alloc retry list");
       VD_HIP(ctx->retry_count.ensure(1), "alloc retry list");
       VD_HIP(vd::fill_u32(ctx->retry_count.p, 1, 0u, ctx->stream), "clear retry count");

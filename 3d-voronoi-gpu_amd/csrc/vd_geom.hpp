@@ -154,6 +154,70 @@ __device__ __forceinline__ void tet_vol_bary(const P4& A, const P4& B, const P4&
   bary.w = 1.f;
 }
 
+// ---- FP64 moments of the robust formulation ----
+//
+// The polytope's topology (which planes bound it, which triples meet at its
+// vertices) comes from the FP32 clipping; its volume and first moment are then
+// integrated in FP64 from planes rebuilt in FP64 from the FP32 inputs: a tet
+// face from the tet's corners, a bisector from its FP32 normal n (= s - z, the
+// subtraction of two nearby FP32 coordinates) and the seed s, as the plane
+// through s - n/2 with normal n, so w = -s.n + n.n/2.  In the reference's
+// [0,1000] frame FP32 offsets of planes through points ~500 units out lose
+// about 1e-4 of a piece's volume; these planes do not.
+struct D4 {
+  double x, y, z, w;
+};
+__device__ __forceinline__ double dot3d(const D4& a, const D4& b) { return fma(a.z, b.z, fma(a.y, b.y, a.x * b.x)); }
+__device__ __forceinline__ D4 sub3d(const D4& a, const D4& b) { return {a.x - b.x, a.y - b.y, a.z - b.z, 0.0}; }
+__device__ __forceinline__ D4 cross3d(const D4& a, const D4& b) {
+  return {msub64(a.y, b.z, a.z, b.y), msub64(a.z, b.x, a.x, b.z), msub64(a.x, b.y, a.y, b.x), 0.0};
+}
+__device__ __forceinline__ D4 up4(const P4& p) { return {(double)p.x, (double)p.y, (double)p.z, (double)p.w}; }
+__device__ __forceinline__ double det3d(double a11, double a12, double a13, double a21, double a22, double a23,
+                                        double a31, double a32, double a33) {
+  const double c1 = msub64(a22, a33, a23, a32);
+  const double c2 = msub64(a12, a33, a13, a32);
+  const double c3 = msub64(a12, a23, a13, a22);
+  return fma(a31, c3, msub64(a11, c1, a21, c2));
+}
+__device__ __forceinline__ D4 face64(const P4& v0, const P4& v1, const P4& v2) {
+  const D4 a = up4(v0);
+  D4 n = cross3d(sub3d(up4(v1), a), sub3d(up4(v2), a));
+  n.w = -dot3d(a, n);
+  return n;
+}
+__device__ __forceinline__ D4 bisector64(const P4& pl, const P4& s) {
+  D4 n = {(double)pl.x, (double)pl.y, (double)pl.z, 0.0};
+  n.w = fma(0.5, dot3d(n, n), -dot3d(up4(s), n));
+  return n;
+}
+__device__ __forceinline__ D4 vertex3d(const D4& p1, const D4& p2, const D4& p3) {
+  const double X = -det3d(p1.w, p1.y, p1.z, p2.w, p2.y, p2.z, p3.w, p3.y, p3.z);
+  const double Y = -det3d(p1.x, p1.w, p1.z, p2.x, p2.w, p2.z, p3.x, p3.w, p3.z);
+  const double Z = -det3d(p1.x, p1.y, p1.w, p2.x, p2.y, p2.w, p3.x, p3.y, p3.w);
+  const double r = 1.0 / det3d(p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p3.x, p3.y, p3.z);
+  return {X * r, Y * r, Z * r, 1.0};
+}
+__device__ __forceinline__ D4 foot_on_plane_d(const D4& P, const D4& pl) {
+  const double lambda = (dot3d(pl, P) + pl.w) / dot3d(pl, pl);
+  return {fma(-lambda, pl.x, P.x), fma(-lambda, pl.y, P.y), fma(-lambda, pl.z, P.z), 1.0};
+}
+__device__ __forceinline__ D4 foot_on_line_d(const D4& P, const D4& X, const D4& u) {
+  const double lambda = dot3d(sub3d(P, X), u) / dot3d(u, u);
+  return {fma(lambda, u.x, X.x), fma(lambda, u.y, X.y), fma(lambda, u.z, X.z), 1.0};
+}
+// Six times the signed volume of tet (A,B,C,D) and four times its barycenter:
+// the 1/6 and 1/4 are applied once to a polytope's sums.
+__device__ __forceinline__ void tet_vol6_bary4_d(const D4& A, const D4& B, const D4& C, const D4& D, double& vol6,
+                                                 D4& bary4) {
+  const D4 a = sub3d(A, D), b = sub3d(B, D), c = sub3d(C, D);
+  vol6 = -det3d(a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z);
+  bary4.x = ((A.x + B.x) + C.x) + D.x;
+  bary4.y = ((A.y + B.y) + C.y) + D.y;
+  bary4.z = ((A.z + B.z) + C.z) + D.z;
+  bary4.w = 1.0;
+}
+
 __device__ __forceinline__ P4 ld4(const float4* p, int i) {
   const float4 v = p[i];
   return {v.x, v.y, v.z, v.w};
@@ -171,6 +235,13 @@ struct PairSums<0> {
     z = fmaf(tv, b.z, z);
     v += tv;
   }
+  // a polytope's sums computed in FP64, rounded to FP32
+  __device__ __forceinline__ void set(double tv, double tx, double ty, double tz) {
+    v = (float)tv;
+    x = (float)tx;
+    y = (float)ty;
+    z = (float)tz;
+  }
 };
 template <>
 struct PairSums<1> {
@@ -180,6 +251,12 @@ struct PairSums<1> {
     y = fma((double)tv, (double)b.y, y);
     z = fma((double)tv, (double)b.z, z);
     v += (double)tv;
+  }
+  __device__ __forceinline__ void set(double tv, double tx, double ty, double tz) {
+    v = tv;
+    x = tx;
+    y = ty;
+    z = tz;
   }
 };
 

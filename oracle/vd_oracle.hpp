@@ -34,6 +34,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <limits>
 #include <vector>
 
@@ -518,10 +519,95 @@ inline bool pair_moments_robust(const Poly<R>& c, A& vol, A& mx, A& my, A& mz) {
   return !(std::fabs((double)vol) < 0.1);
 }
 
-// Moment formulation selector: 0 = the reference's, 1 = robust.
+// The robust formulation of an FP32 polytope, integrated in FP64 (this
+// build's own formulation, as the HIP kernels compute it): the topology (planes,
+// vertex triples) is the FP32 clipping's; each plane is rebuilt in FP64 from
+// the FP32 inputs -- a tet face from the tet's corners, a bisector from its
+// FP32 normal n (= s - z) and the seed s as the plane through s - n/2
+// (w = -s.n + n.n/2), a box plane as it is -- so that the vertices, volumes
+// and moments carry no FP32 rounding of the [0,1000] frame.  cell_nv < 0: a
+// tet -> cell polytope (ids 0-3 the faces, later ids bisectors); cell_nv >= 0:
+// a cell clipped by a tet (ids 0-5 the box, ids below cell_nv bisectors, later
+// ids the tet's faces, recognised by their FP32 planes).
+template <class R>
+inline V4<double> up64(const V4<R>& q) {
+  return {(double)q.x, (double)q.y, (double)q.z, (double)q.w};
+}
+template <class R>
+inline V4<double> plane64(const Poly<R>& c, int id, const V4<R> tv[4], int cell_nv) {
+  static const int face[4][3] = {{2, 1, 3}, {0, 2, 3}, {1, 0, 3}, {0, 1, 2}};
+  const V4<R>& p = c.pl[id];
+  auto bis = [&]() {
+    V4<double> n = {(double)p.x, (double)p.y, (double)p.z, 0.0};
+    n.w = std::fma(0.5, dot3(n, n), -dot3(up64(c.seed), n));
+    return n;
+  };
+  int f = id;
+  if (cell_nv < 0) {
+    if (id >= 4) return bis();
+  } else {
+    if (id < 6) return up64(p);
+    if (id < cell_nv) return bis();
+    f = -1;
+    for (int k = 0; k < 4 && f < 0; ++k) {
+      const V4<R> q = tri2plane(tv[face[k][0]], tv[face[k][1]], tv[face[k][2]]);
+      if (std::memcmp(&q, &p, sizeof q) == 0) f = k;
+    }
+    if (f < 0) return up64(p);
+  }
+  return tri2plane(up64(tv[face[f][0]]), up64(tv[face[f][1]]), up64(tv[face[f][2]]));
+}
+inline V4<double> vertex3d(const V4<double>& p1, const V4<double>& p2, const V4<double>& p3) {
+  const double X = -det3(p1.w, p1.y, p1.z, p2.w, p2.y, p2.z, p3.w, p3.y, p3.z);
+  const double Y = -det3(p1.x, p1.w, p1.z, p2.x, p2.w, p2.z, p3.x, p3.w, p3.z);
+  const double Z = -det3(p1.x, p1.y, p1.w, p2.x, p2.y, p2.w, p3.x, p3.y, p3.w);
+  const double r = 1.0 / det3(p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, p3.x, p3.y, p3.z);
+  return {X * r, Y * r, Z * r, 1.0};
+}
 template <class R, class A>
-inline bool moments(const Poly<R>& c, int formulation, A& vol, A& mx, A& my, A& mz) {
-  return formulation == 1 ? pair_moments_robust<R, A>(c, vol, mx, my, mz) : pair_moments<R, A>(c, vol, mx, my, mz);
+inline bool pair_moments_robust64(const Poly<R>& c, const V4<R> tv[4], int cell_nv, A& vol, A& mx, A& my, A& mz) {
+  // apex: the mean of the FP32 vertices (the signed ring is exact for any apex)
+  V4<R> Of = {R(0), R(0), R(0), R(1)};
+  for (int t = 0; t < c.nt; ++t) {
+    const V4<R> X = c.vertex(t);
+    Of.x += X.x; Of.y += X.y; Of.z += X.z;
+  }
+  const R inv = R(1) / R(c.nt > 0 ? c.nt : 1);
+  Of.x *= inv; Of.y *= inv; Of.z *= inv;
+  const V4<double> O = up64(Of);
+  double sv = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;  // 6 x volume, 24 x moments
+  V4<double> P[6];
+  for (int t = 0; t < c.nt; ++t) {
+    const V4<double> q[3] = {plane64(c, c.tr[t][0], tv, cell_nv), plane64(c, c.tr[t][1], tv, cell_nv),
+                             plane64(c, c.tr[t][2], tv, cell_nv)};
+    const V4<double> X = vertex3d(q[0], q[1], q[2]);
+    for (int i = 0; i < 3; ++i) P[2 * i] = foot_on_plane(O, q[i]);
+    for (int i = 0; i < 3; ++i) P[2 * i + 1] = foot_on_line(O, X, cross3(q[i], q[(i + 1) % 3]));
+    for (int i = 0; i < 6; ++i) {
+      const V4<double>& Ap = P[i];
+      const V4<double>& Bp = P[(i + 1) % 6];
+      const V4<double> a = sub3(Ap, X), b = sub3(Bp, X), cc = sub3(O, X);
+      const double vol6 = -det3(a.x, a.y, a.z, b.x, b.y, b.z, cc.x, cc.y, cc.z);
+      sx = std::fma(vol6, ((Ap.x + Bp.x) + O.x) + X.x, sx);
+      sy = std::fma(vol6, ((Ap.y + Bp.y) + O.y) + X.y, sy);
+      sz = std::fma(vol6, ((Ap.z + Bp.z) + O.z) + X.z, sz);
+      sv += vol6;
+    }
+  }
+  vol = (A)(sv / 6.0); mx = (A)(sx / 24.0); my = (A)(sy / 24.0); mz = (A)(sz / 24.0);
+  if (c.ev) { c.ev->acc_tris += c.nt; c.ev->acc_pairs += 1; }
+  return !(std::fabs((double)vol) < 0.1);
+}
+
+// Moment formulation selector: 0 = the reference's, 1 = robust (an FP32
+// polytope's integrated in FP64 as above; an FP64 polytope's in its own
+// arithmetic).
+template <class R, class A>
+inline bool moments(const Poly<R>& c, int formulation, const V4<R> tv[4], int cell_nv, A& vol, A& mx, A& my,
+                    A& mz) {
+  if (formulation != 1) return pair_moments<R, A>(c, vol, mx, my, mz);
+  if constexpr (std::is_same<R, float>::value) return pair_moments_robust64<R, A>(c, tv, cell_nv, vol, mx, my, mz);
+  return pair_moments_robust<R, A>(c, vol, mx, my, mz);
 }
 
 // ---------------------------------------------------------------- kNN ------
@@ -561,9 +647,9 @@ struct PairOut {
 // in a degenerate configuration) is not accumulated and reported as
 // needs_exact_predicates -- the reference would add the NaN.
 template <class R, class A>
-inline void finish_pair(Poly<R>& c, int mform, PairOut& o) {
+inline void finish_pair(Poly<R>& c, int mform, const V4<R> tv[4], int cell_nv, PairOut& o) {
   A vol, mx, my, mz;
-  const bool keep = moments<R, A>(c, mform, vol, mx, my, mz);
+  const bool keep = moments<R, A>(c, mform, tv, cell_nv, vol, mx, my, mz);
   if (!(std::isfinite((double)vol) && std::isfinite((double)mx) && std::isfinite((double)my) &&
         std::isfinite((double)mz))) {
     c.status = needs_exact_predicates;
@@ -597,7 +683,7 @@ inline PairOut clip_pair_tet(Poly<R>& c, const float* sites, const int32_t* nbr,
     }
   }
   if (k == 0 || !c.security_reached(site(nbr[k - 1]))) c.status = security_radius_not_reached;
-  if (c.status != no_intersection) finish_pair<R, A>(c, mform, o);
+  if (c.status != no_intersection) finish_pair<R, A>(c, mform, tv, -1, o);
   o.status = c.status;
   return o;
 }
@@ -625,11 +711,12 @@ template <class R, class A>
 inline PairOut clip_pair_cell(Poly<R>& c, const V4<R> tv[4], int mform) {
   static const int face[4][3] = {{2, 1, 3}, {0, 2, 3}, {1, 0, 3}, {0, 1, 2}};
   PairOut o{};
+  const int cell_nv = c.nv;
   for (int i = 0; i < 4; ++i) {
     c.clip(tri2plane(tv[face[i][0]], tv[face[i][1]], tv[face[i][2]]), true);
     if (c.status == no_intersection) break;
   }
-  if (c.status != no_intersection) finish_pair<R, A>(c, mform, o);
+  if (c.status != no_intersection) finish_pair<R, A>(c, mform, tv, cell_nv, o);
   o.status = c.status;
   return o;
 }
