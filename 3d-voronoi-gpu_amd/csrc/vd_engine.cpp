@@ -54,6 +54,13 @@ struct DevBuf {
 // recomputed at the full depth.
 constexpr int kAutoSiteDepth = 40;
 int site_depth_of(const vd_config& c, int K) { return std::min(c.site_depth > 0 ? c.site_depth : kAutoSiteDepth, K); }
+// The automatic depth of the first clipping after the caller sets sites.  Such
+// sites have not been moved by a Lloyd update yet (random samples, typically):
+// their cells are irregular and many more pairs need over 40 bisectors (336k
+// of 63.9M for 100k uniform sites against none after a few updates), whose
+// re-clip from scratch cost more than the deeper tables' lost occupancy (one
+// wave per CU fewer at the same vertex cap).
+constexpr int kFreshSiteDepth = 64;
 
 int tet_k_heuristic(int n_site, int n_tet, int preferred) {
   // voronoi.cu:936
@@ -145,6 +152,9 @@ struct vd_ctx {
   // kNN step bounds each query by its previous row
   bool rows_valid = false;
   int rows_K = 0, rows_TK = 0;
+  // sites set by the caller and not clipped since (the next site-major
+  // clipping at the automatic depth uses kFreshSiteDepth)
+  bool fresh_sites = false;
   // site-major pairs
   DevBuf<int> p_count, p_cursor;
   DevBuf<unsigned> p_start, p_ids;
@@ -572,6 +582,7 @@ int vd_set_batch(vd_ctx* ctx, int n_inst, const int32_t* n_verts, const int32_t*
   rc = ensure_grid(ctx, std::max(ctx->n_sites, 1));
   if (rc != VD_OK) return rc;
   ctx->have_knn = ctx->have_clip = false;
+  ctx->fresh_sites = true;
   return VD_OK;
 }
 
@@ -597,6 +608,7 @@ int vd_set_sites(vd_ctx* ctx, const float* sites_xyz, int n_sites) {
   if (rc != VD_OK) return rc;
   ctx->n_sites = n_sites;
   ctx->have_knn = ctx->have_clip = false;
+  ctx->fresh_sites = true;
   return VD_OK;
 }
 
@@ -771,9 +783,14 @@ int vd_clip_step(vd_ctx* ctx) {
     a.site_counter = ctx->site_counter.p;
     const int fast_tcap = vdi::site_tcap_of(c);
     a.site_depth = site_depth_of(c, ctx->K);
+    if (ctx->fresh_sites && c.site_depth == 0) a.site_depth = std::min(kFreshSiteDepth, ctx->K);
+    ctx->fresh_sites = false;
     // (the robust moments also need it: the site-major kernel retries the
-    // polytopes it cannot hand to the volume pass)
-    if (fast_tcap < c.tcap || a.site_depth < ctx->K || c.moments == VD_MOMENTS_ROBUST) {
+    // polytopes it cannot hand to the volume pass; and it is allocated alike
+    // at either depth, so that an iteration captured in a graph after a first
+    // eager one allocates nothing)
+    if (fast_tcap < c.tcap || std::min(a.site_depth, site_depth_of(c, ctx->K)) < ctx->K ||
+        c.moments == VD_MOMENTS_ROBUST) {
       VD_HIP(ctx->retry_list.ensure(std::max(n_pairs, 1LL)), "alloc retry list");
       VD_HIP(ctx->retry_count.ensure(1), "alloc retry list");
       VD_HIP(vd::fill_u32(ctx->retry_count.p, 1, 0u, ctx->stream), "clear retry count");
@@ -783,7 +800,7 @@ int vd_clip_step(vd_ctx* ctx) {
     a.tcap = fast_tcap;
     a.tcap_full = c.tcap;
     a.cull_depth = c.cull_depth;
-    if (std::min(c.cull_depth, site_depth_of(c, ctx->K)) > 0) {
+    if (std::min(c.cull_depth, a.site_depth) > 0) {
       // the emptiness test as a pass of its own; the clipping takes the survivors
       VD_HIP(ctx->p_surv.ensure(n), "alloc pairs");
       VD_HIP(vd::cull_sites(a, ctx->p_ids.p, ctx->p_surv.p, ctx->stream), "cull");

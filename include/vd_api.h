@@ -96,7 +96,9 @@ typedef struct vd_config {
                          0 = automatic (n_pairs / 10 + 262144), -1 = every volume in place */
   int site_depth;     /* site-major form: bisectors per site table of the launch; a pair that
                          needs more is recomputed from scratch at site_k by the retry pass,
-                         so results do not depend on it; 0 = automatic (40) */
+                         so results do not depend on it; 0 = automatic (40, and 64 for the
+                         first clipping after vd_set_sites or vd_set_batch: sites not yet
+                         moved by an update need deeper tables) */
   int knn_waves;      /* wavefronts per workgroup of the kNN queries: 2, 4 or 8; 0 = automatic
                          (4); results do not depend on it */
 } vd_config;

@@ -146,6 +146,31 @@ def test_block_sizes_keep_results(vdn, knn_waves, clip_blocks):
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("moments", ["robust", "reference"])
+def test_fresh_site_depth_keeps_results(vdn, moments):
+    """The first site-major clipping after set_sites uses 64-bisector tables
+    (automatic depth), later ones 40: the sites of every iteration, and of a
+    chain restarted by set_sites, equal those at a fixed depth of 40 and of
+    site_k (no pair recomputed deeper) bit for bit."""
+    from voronoi_amd import meshgen
+
+    v, t, s = meshgen.make_problem("cube", 16, 4000, seed=11)
+    runs = []
+    for depth in (0, 40, 90):
+        with vdn.Engine(accum="f64", moments=moments, tet_k=64, site_k=90, site_depth=depth) as e:
+            e.set_mesh(v, t)
+            got = []
+            for _ in range(2):  # the second chain restarts from the same sites
+                e.set_sites(s)
+                for _ in range(3):
+                    e.lloyd(1)
+                    got.append(e.sites())
+            runs.append(got)
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("tol", [0.0, 1e-5])
 def test_autotune_k(vdn, tol):
     """vd_autotune_k keeps only smaller neighbour/candidate counts whose sites
