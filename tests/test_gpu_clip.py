@@ -178,19 +178,32 @@ def test_site_form_depth_retry(vdn, O, depth):
     assert [g["hist"][k] for k in vdn.STATUS_NAMES] == hist.tolist()
 
 
-def test_accuracy_vs_fp64_oracle(vdn, O):
-    """FP32 clipping + FP64 accumulation vs the all-FP64 oracle."""
+@pytest.mark.parametrize("moments", ["robust", "reference"])
+def test_accuracy_vs_fp64_oracle(vdn, O, moments):
+    """FP32 clipping + FP64 accumulation vs the all-FP64 oracle (48,000 tets,
+    2,000 sites, TK 64).  The robust decomposition, integrated in FP64 over
+    planes rebuilt in FP64, meets the 1e-5 bar on cell volume, total volume
+    and centroid with a wide margin.  The reference's decomposition (FP32
+    pieces around the seed, foot projections skipped when |n|^2 <= 1e-2)
+    misses it on cell volumes (here by 3.8e-2, with the total volume off by
+    1.7e-4): that is why it is not the headline form."""
     verts, tets, sites = problem("cube", 20, 2000)
     sk, tk = oracle_knn("cube", 20, 2000, 90, 64)
-    g = _run(vdn, verts, tets, sites, tet_k=64, accum="f64", moments="robust")
+    g = _run(vdn, verts, tets, sites, tet_k=64, accum="f64", moments=moments)
     r = O.clip_tets(sites, sk, verts, tets, tk, geom="f64", pair_acc="f64", site_mode=O.SITE_F64,
                     moments="robust", want_pairs=False)
     a, b = g["moments"], r["site_vals"]
     vol_err = np.abs(a[:, 0] - b[:, 0]) / b[:, 0]
+    tot_err = abs(a[:, 0].sum() - b[:, 0].sum()) / b[:, 0].sum()
     cen_err = np.linalg.norm(a[:, 1:] / a[:, :1] - b[:, 1:] / b[:, :1], axis=1) / np.linalg.norm(b[:, 1:] / b[:, :1], axis=1)
-    print("volume rel err max/mean", vol_err.max(), vol_err.mean(), "centroid rel err max/mean", cen_err.max(), cen_err.mean())
-    assert vol_err.max() <= 1e-5
-    assert cen_err.max() <= 1e-5
+    print(moments, "volume rel err max/mean", vol_err.max(), vol_err.mean(), "total", tot_err,
+          "centroid rel err max/mean", cen_err.max(), cen_err.mean())
+    if moments == "robust":
+        assert vol_err.max() <= 1e-6
+        assert tot_err <= 1e-9
+        assert cen_err.max() <= 1e-6
+    else:
+        assert vol_err.max() > 1e-5
 
 
 def test_single_site_takes_whole_domain(vdn, O):
