@@ -210,6 +210,11 @@ struct Cell {
   // ball of the current bisector (see clip), and the d2z of the last clip
   uint32_t far;
   float last_d2z;
+  // cached form: the largest squared vertex distance to the seed (fmaxf from
+  // 0, as the security test takes it), and the tet's bounding box relative to
+  // the seed (every vertex of the polytope lies in it)
+  float vmax;
+  float blo[3], bhi[3];
 
   __device__ __forceinline__ bool ok() const {
     return status == kSuccess || (keep_srnr && status == kSecurityRadiusNotReached);
@@ -275,14 +280,24 @@ struct Cell {
     keep_srnr = false;
     put_tet_faces(v);
     nv = 4;
-    put_tri(0, pack3(1, 3, 2));
-    put_tri(1, pack3(0, 1, 2));
-    put_tri(2, pack3(0, 3, 1));
-    put_tri(3, pack3(0, 2, 3));
+    float m = 0.f;
+    m = fmaxf(put_tri(0, pack3(1, 3, 2)), m);
+    m = fmaxf(put_tri(1, pack3(0, 1, 2)), m);
+    m = fmaxf(put_tri(2, pack3(0, 3, 1)), m);
+    m = fmaxf(put_tri(3, pack3(0, 2, 3)), m);
+    vmax = m;
     nt = 4;
     first = kNoPlane;
     far = 0xFu;
     last_d2z = 0.f;
+    if constexpr (TS::kCached) {
+      blo[0] = fminf(fminf(v[0].x, v[1].x), fminf(v[2].x, v[3].x)) - s.x;
+      blo[1] = fminf(fminf(v[0].y, v[1].y), fminf(v[2].y, v[3].y)) - s.y;
+      blo[2] = fminf(fminf(v[0].z, v[1].z), fminf(v[2].z, v[3].z)) - s.z;
+      bhi[0] = fmaxf(fmaxf(v[0].x, v[1].x), fmaxf(v[2].x, v[3].x)) - s.x;
+      bhi[1] = fmaxf(fmaxf(v[0].y, v[1].y), fmaxf(v[2].y, v[3].y)) - s.y;
+      bhi[2] = fmaxf(fmaxf(v[0].z, v[1].z), fmaxf(v[2].z, v[3].z)) - s.z;
+    }
   }
 
   // Box start of kernel B: planes x >= -0.1, x <= 1000.1, y.., z.. and the
@@ -384,8 +399,8 @@ struct Cell {
   // at once.  clip_test: the plane-cap check, the conflict flags and the
   // security outcome; returns as clip() when the plane removes no vertex (or
   // flags needs_exact_predicates, or overflows), else 2 with the flags in
-  // conf_out and the security outcome of the kept vertices in secure_out.
-  __device__ __forceinline__ int clip_test(const P4& eqn, float d2z, uint32_t& conf_out, bool& secure_out) {
+  // conf_out.
+  __device__ __forceinline__ int clip_test(const P4& eqn, float d2z, uint32_t& conf_out) {
     if (nv >= pcap) {
       status = kVertexOverflow;
       return -1;
@@ -398,13 +413,26 @@ struct Cell {
     // without computing it.  d2z does not decrease along the neighbour list,
     // so a vertex found inside that ball stays inside for the later
     // bisectors: `far` keeps the slots not yet found inside.
-    // Security: a vertex blocks it when 4 d2 >= d2z (false for NaN, which
-    // the reference's fmaxf skips); only far vertices can block.
+    // The same margin decides a whole bisector at once: the plane lies at
+    // |z - s| / 2 from the seed along u = (z - s) / |z - s|, so when every
+    // point of the tet's box (which holds every vertex) projects below
+    // 0.45 |z - s| on u, no vertex can be in conflict.  Then only the security
+    // outcome is needed, and the largest vertex distance gives it.
+    if (!filter) {
+      // (z - s) = -e.xyz; max over the box of (x - s).(z - s)
+      const float px = -fminf(blo[0] * eqn.x, bhi[0] * eqn.x);
+      const float py = -fminf(blo[1] * eqn.y, bhi[1] * eqn.y);
+      const float pz = -fminf(blo[2] * eqn.z, bhi[2] * eqn.z);
+      if (px + py + pz < 0.45f * d2z) {
+        --nv;
+        return d2z > 4.f * vmax ? 1 : 0;
+      }
+    }
     if (!(d2z >= last_d2z)) far = (1u << nt) - 1u;
     last_d2z = d2z;
     // (with the filter on every determinant is evaluated, as in the reference)
     const float near2 = filter ? -1.f : 0.2025f * d2z;
-    uint32_t cand = 0, block = 0;
+    uint32_t cand = 0;
     // kFarUnroll slots per loop trip, so that their LDS reads are in flight
     // together
     for (uint32_t m = far; m;) {
@@ -427,7 +455,6 @@ struct Cell {
         const uint32_t keep = d[j] < near2 ? 0u : bit;
         far &= ~(bit & ~keep);
         cand |= keep;
-        block |= 4.f * d[j] >= d2z ? keep : 0u;
       }
     }
     // the determinant only for candidates: a wave pays for the largest
@@ -460,7 +487,6 @@ struct Cell {
         }
       }
     }
-    const bool kept_secure = d2z > 0.f && (block & ~conf) == 0u;
     if (status == kNeedsExactPredicates) {
       // the reference stops right after its conflict loop (voronoi.cu:459-461):
       // the conflicting vertices are gone, no plane is dropped, no fan
@@ -482,16 +508,15 @@ struct Cell {
     }
     if (conf == 0u) {
       --nv;
-      return kept_secure ? 1 : 0;
+      return d2z > 4.f * vmax ? 1 : 0;
     }
     conf_out = conf;
-    secure_out = kept_secure;
     return 2;
   }
 
   // clip_cut: removal of the flagged vertices, the cavity walk and the fan of
   // new vertices around plane `pid`; returns as clip().
-  __device__ __forceinline__ int clip_cut(int pid, uint32_t conf, bool kept_secure, float d2z) {
+  __device__ __forceinline__ int clip_cut(int pid, uint32_t conf, float d2z) {
     const float near2 = filter ? -1.f : 0.2025f * d2z;
     // The reference's loop (ascending slot, a conflicting slot takes the
     // last live triangle and is tested again) replayed from the flags: a
@@ -512,10 +537,16 @@ struct Cell {
       status = kNoIntersection;
       return -1;
     }
+    // the largest distance of the kept vertices, then of the new ones (the
+    // security test: every vertex closer than half of |z - s|)
+    float m = 0.f;
+    for (int t = 0; t < nt; ++t) m = fmaxf(ts.d2(t), m);
     cavity_boundary();
     if (status != kSuccess) return -1;
-    if (first == kNoPlane) return kept_secure ? 1 : 0;
-    bool new_block = false;
+    if (first == kNoPlane) {
+      vmax = m;
+      return d2z > 4.f * m ? 1 : 0;
+    }
     int c = first;
     int guard = 0;
     // the new vertex (pid, c, n): the plane of pid and of c are already in
@@ -535,7 +566,7 @@ struct Cell {
       const float d = seed_d2(vertex3(pp, pc, pn));
       ts.put(nt, pack3(pid, c, n), d);
       far |= d < near2 ? 0u : 1u << nt;
-      new_block |= 4.f * d >= d2z;
+      m = fmaxf(d, m);
       ++nt;
       c = n;
       pc = pn;
@@ -544,7 +575,8 @@ struct Cell {
       status = err;
       return -1;
     }
-    return (kept_secure && !new_block) ? 1 : 0;
+    vmax = m;
+    return d2z > 4.f * m ? 1 : 0;
   }
 
   // Clip by plane eqn stored under id `pid` (voronoi.cu:443-486); the plane is
@@ -556,9 +588,8 @@ struct Cell {
   __device__ int clip(int pid, const P4& eqn, float d2z) {
     if constexpr (TS::kCached) {
       uint32_t conf = 0;
-      bool kept_secure = false;
-      const int r = clip_test(eqn, d2z, conf, kept_secure);
-      return r != 2 ? r : clip_cut(pid, conf, kept_secure, d2z);
+      const int r = clip_test(eqn, d2z, conf);
+      return r != 2 ? r : clip_cut(pid, conf, d2z);
     } else {
       if (nv >= pcap) {
         status = kVertexOverflow;
@@ -780,7 +811,7 @@ __device__ __forceinline__ bool run_pair(const ClipArgs& a, Cell<PS, TS>& c, con
     int v = 0;
     bool go = K > 0;
     while (go) {
-      bool cut = false, ksec = false;
+      bool cut = false;
       uint32_t conf = 0;
       float cut_d2z = 0.f;
       int cut_pid = 0;
@@ -794,7 +825,7 @@ __device__ __forceinline__ bool run_pair(const ClipArgs& a, Cell<PS, TS>& c, con
         P4 eq;
         float d2z;
         plane_of(v, z, pid, eq, d2z);
-        const int r = c.clip_test(eq, d2z, conf, ksec);
+        const int r = c.clip_test(eq, d2z, conf);
         if (r == 2) {
           cut = true;
           cut_d2z = d2z;
@@ -811,7 +842,7 @@ __device__ __forceinline__ bool run_pair(const ClipArgs& a, Cell<PS, TS>& c, con
         }
       }
       if (cut) {
-        const int r = c.clip_cut(cut_pid, conf, ksec, cut_d2z);
+        const int r = c.clip_cut(cut_pid, conf, cut_d2z);
         if (r >= 0 ? r != 0 : c.security_reached(cut_d2z)) {
           stopped = true;
           go = false;
@@ -1178,7 +1209,6 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
       // the others wait
       bool cut = false;
       uint32_t conf = 0;
-      bool ksec = false;
       float cut_d2z = 0.f;
       // The loop's exits after the clip of bisector pv: 0 go on, 1 finished
       // (the volume pass follows), 2 ended without one, 3 recomputed deeper.
@@ -1207,7 +1237,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
             out = 2;
             break;
           }
-          const int r = c.clip_test(e, d2z, conf, ksec);
+          const int r = c.clip_test(e, d2z, conf);
           if (r == 2) {
             cut = true;
             cut_d2z = d2z;
@@ -1216,7 +1246,7 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
           out = step_end(r, d2z);
         }
       }
-      if (cut) out = step_end(c.clip_cut(4 + pv, conf, ksec, cut_d2z), cut_d2z);
+      if (cut) out = step_end(c.clip_cut(4 + pv, conf, cut_d2z), cut_d2z);
       if (out == 3) {
         const unsigned slot = atomicAdd(a.retry_count, 1u);
         if (slot < (unsigned)a.n_pairs) a.retry_list[slot] = ppair;
