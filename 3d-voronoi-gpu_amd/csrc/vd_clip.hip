@@ -193,6 +193,14 @@ struct TriRecord {
   __device__ __forceinline__ uint32_t get(int t) const { return tr[t]; }
 };
 
+// Corners of a tet from its record (x0 y0 z0 x1 | y1 z1 x2 y2 | z2 x3 y3 z3).
+__device__ __forceinline__ void corners_of(const float4& q0, const float4& q1, const float4& q2, P4 v[4]) {
+  v[0] = {q0.x, q0.y, q0.z, 0.f};
+  v[1] = {q0.w, q1.x, q1.y, 0.f};
+  v[2] = {q1.z, q1.w, q2.x, 0.f};
+  v[3] = {q2.y, q2.z, q2.w, 0.f};
+}
+
 template <class PS, class TS>
 struct Cell {
   PS ps;
@@ -297,6 +305,51 @@ struct Cell {
       bhi[0] = fmaxf(fmaxf(v[0].x, v[1].x), fmaxf(v[2].x, v[3].x)) - s.x;
       bhi[1] = fmaxf(fmaxf(v[0].y, v[1].y), fmaxf(v[2].y, v[3].y)) - s.y;
       bhi[2] = fmaxf(fmaxf(v[0].z, v[1].z), fmaxf(v[2].z, v[3].z)) - s.z;
+    }
+  }
+
+  // The same start from a tet's precomputed record (tet_prep: the four face
+  // planes, then the vertices of triangles 0..3 as x0 y0 z0 x1 | y1 z1 x2 y2 |
+  // z2 x3 y3 z3), which holds what put_tet_faces and put_tri compute.
+  __device__ void init_tet_prep(const P4& s, const float4* rec) {
+    seed = s;
+    status = kSuccess;
+    keep_srnr = false;
+    const float4 f0 = rec[0], f1 = rec[1], f2 = rec[2], f3 = rec[3];
+    ps.put(0, {f0.x, f0.y, f0.z, f0.w});
+    ps.put(1, {f1.x, f1.y, f1.z, f1.w});
+    ps.put(2, {f2.x, f2.y, f2.z, f2.w});
+    ps.put(3, {f3.x, f3.y, f3.z, f3.w});
+    nv = 4;
+    nt = 4;
+    first = kNoPlane;
+    far = 0xFu;
+    last_d2z = 0.f;
+    if constexpr (TS::kCached) {
+      const float4 q0 = rec[4], q1 = rec[5], q2 = rec[6];
+      P4 X[4];
+      corners_of(q0, q1, q2, X);
+      const uint32_t tri[4] = {pack3(1, 3, 2), pack3(0, 1, 2), pack3(0, 3, 1), pack3(0, 2, 3)};
+      float m = 0.f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float d2 = seed_d2(X[k]);
+        ts.put(k, tri[k], d2);
+        m = fmaxf(d2, m);
+      }
+      vmax = m;
+      blo[0] = fminf(fminf(X[0].x, X[1].x), fminf(X[2].x, X[3].x)) - s.x;
+      blo[1] = fminf(fminf(X[0].y, X[1].y), fminf(X[2].y, X[3].y)) - s.y;
+      blo[2] = fminf(fminf(X[0].z, X[1].z), fminf(X[2].z, X[3].z)) - s.z;
+      bhi[0] = fmaxf(fmaxf(X[0].x, X[1].x), fmaxf(X[2].x, X[3].x)) - s.x;
+      bhi[1] = fmaxf(fmaxf(X[0].y, X[1].y), fmaxf(X[2].y, X[3].y)) - s.y;
+      bhi[2] = fmaxf(fmaxf(X[0].z, X[1].z), fmaxf(X[2].z, X[3].z)) - s.z;
+    } else {
+      ts.put(0, pack3(1, 3, 2), 0.f);
+      ts.put(1, pack3(0, 1, 2), 0.f);
+      ts.put(2, pack3(0, 3, 1), 0.f);
+      ts.put(3, pack3(0, 2, 3), 0.f);
+      vmax = 0.f;
     }
   }
 
@@ -783,13 +836,6 @@ __device__ __forceinline__ void count_status(const ClipArgs& a, int status, bool
 
 __device__ __forceinline__ P4 site_at(const ClipArgs& a, int i) { return ld4(a.sites, i); }
 
-// Corners of a tet from its record (x0 y0 z0 x1 | y1 z1 x2 y2 | z2 x3 y3 z3).
-__device__ __forceinline__ void corners_of(const float4& q0, const float4& q1, const float4& q2, P4 v[4]) {
-  v[0] = {q0.x, q0.y, q0.z, 0.f};
-  v[1] = {q0.w, q1.x, q1.y, 0.f};
-  v[2] = {q1.z, q1.w, q2.x, 0.f};
-  v[3] = {q2.y, q2.z, q2.w, 0.f};
-}
 __device__ __forceinline__ void tet_corners(const ClipArgs& a, int t, P4 v[4]) {
   const float4* r = a.tet_xyz + 3 * (size_t)t;
   corners_of(r[0], r[1], r[2], v);
@@ -905,6 +951,26 @@ __device__ __forceinline__ bool run_pair(const ClipArgs& a, Cell<PS, TS>& c, con
   return true;
 }
 
+// Per tet: its face planes and the vertices of its four start triangles, as
+// put_tet_faces and put_tri compute them (the same functions, so the same
+// bits), for init_tet_prep.
+__global__ __launch_bounds__(256) void k_tet_prep(const float4* tet_xyz, long long n, float4* out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  P4 v[4];
+  corners_of(tet_xyz[3 * t], tet_xyz[3 * t + 1], tet_xyz[3 * t + 2], v);
+  const P4 pl[4] = {tri2plane(v[2], v[1], v[3]), tri2plane(v[0], v[2], v[3]), tri2plane(v[1], v[0], v[3]),
+                    tri2plane(v[0], v[1], v[2])};
+  const P4 X[4] = {vertex3(pl[1], pl[3], pl[2]), vertex3(pl[0], pl[1], pl[2]), vertex3(pl[0], pl[3], pl[1]),
+                   vertex3(pl[0], pl[2], pl[3])};
+  float4* o = out + kTetPrepWords * t;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) o[k] = make_float4(pl[k].x, pl[k].y, pl[k].z, pl[k].w);
+  o[4] = make_float4(X[0].x, X[0].y, X[0].z, X[1].x);
+  o[5] = make_float4(X[1].y, X[1].z, X[2].x, X[2].y);
+  o[6] = make_float4(X[2].z, X[3].x, X[3].y, X[3].z);
+}
+
 // ------------------------------------------------------------ tet-major ----
 
 // CACHED: vertices stored with their distance to the seed (at most 32 per
@@ -941,10 +1007,8 @@ __global__ __launch_bounds__(64) void k_clip_plain(ClipArgs a) {
   } else if (live) {
     const int t = (int)(pair / a.TK);
     seed_id = a.tet_knn[pair];
-    const int4 tv = a.tets[t];
-    const P4 v[4] = {ld4(a.verts, tv.x), ld4(a.verts, tv.y), ld4(a.verts, tv.z), ld4(a.verts, tv.w)};
     const P4 s = site_at(a, seed_id);
-    c.init_tet(s, v);
+    c.init_tet_prep(s, a.tet_prep + kTetPrepWords * (size_t)t);
     const int32_t* nbr = a.site_knn + (size_t)seed_id * a.K;
     ClipArgs ak = a;
     if (a.knn2) {
@@ -1137,10 +1201,8 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
       pv = 0;
       lst = kClipping;
       ptet = ppair / (unsigned)a.TK;
-      P4 v[4];
-      tet_corners(a, (int)ptet, v);
       c.ps.bis = bis + pslot * K4;
-      c.init_tet(site_at(a, pslot ? site1 : site0), v);
+      c.init_tet_prep(site_at(a, pslot ? site1 : site0), a.tet_prep + kTetPrepWords * (size_t)ptet);
     }
     qhead += min((unsigned)__popcll(idle), avail);
     const int n_clip = __popcll(__ballot(lst == kClipping));
@@ -2096,6 +2158,8 @@ static hipError_t launch_site(Kern k, int n_blocks, size_t lds, hipStream_t st, 
 hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, int n_blocks, hipStream_t st,
                        bool volume_pass) {
   if (a.n_pairs <= 0) return hipSuccess;
+  // the tet-major and site-major kernels start every pair from its tet's record
+  if ((form == 0 || form == kFormTetCached || form == 1) && !a.tet_prep) return hipErrorInvalidValue;
   if (form == 0) {
     const size_t lds = clip_plain_lds(a.pcap, a.tcap);
     const dim3 grid(a.pair_list ? (unsigned)n_blocks : (unsigned)((a.n_pairs + 63) / 64)), block(64);
@@ -2146,6 +2210,12 @@ hipError_t site_volume_pass(const ClipArgs& a, int moments, int acc_mode, hipStr
     hipLaunchKernelGGL((k_site_moments<1, 0>), grid, block, 0, st, a);
   else
     hipLaunchKernelGGL((k_site_moments<1, 1>), grid, block, 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t tet_prep(const float4* tet_xyz, long long n, float4* out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tet_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, tet_xyz, n, out);
   return hipGetLastError();
 }
 

@@ -13,6 +13,8 @@ struct ClipArgs {
   const float4* verts;       // (x, y, z, _)
   const int4* tets;          // 4 vertex ids, positively oriented
   const float4* tet_xyz;     // site-major form: per tet its corners, 3 float4 (12 floats)
+  const float4* tet_prep;    // optional (tet_prep()): per tet its 4 face planes and the 4
+                             // vertices they make, as the tet start computes them, 7 float4
   const int32_t* tet_knn;    // n_tets x TK candidate sites
   int TK;
   long long n_pairs;         // n_tets * TK
@@ -109,6 +111,12 @@ hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites,
 // range and their number written to pair_count[s]; a culled pair counts as
 // no_intersection (a.status_hist, a.pair_status, a.pair_vals).
 hipError_t cull_sites(const ClipArgs& a, unsigned* pair_ids, unsigned* pair_count, hipStream_t st);
+// Per tet of the corner records tet_xyz[0..3n): its face planes (ids 0..3) and
+// the FP32 vertices of its four start triangles, 7 float4 per tet
+// (kTetPrepWords), so that a pair's start loads them instead of computing
+// them; the values are the ones the start computes, bit for bit.
+constexpr int kTetPrepWords = 7;
+hipError_t tet_prep(const float4* tet_xyz, long long n, float4* out, hipStream_t st);
 
 // Rows of the deeper neighbour lists: every site seeding a listed pair gets a
 // row (row_site[r] = s, row_of[s] = r).  row_of must be all -1 and *n_rows 0.

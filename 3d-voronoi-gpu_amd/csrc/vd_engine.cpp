@@ -136,6 +136,7 @@ struct vd_ctx {
 
   DevBuf<float4> verts, sites;
   DevBuf<float4> tet_xyz;  // per tet its 12 corner coordinates (3 float4)
+  DevBuf<float4> tet_prep; // per tet its face planes and start vertices (vd::tet_prep)
   DevBuf<int4> tets;
   DevBuf<int4> offsets;
   int n_offsets = 0;
@@ -419,7 +420,7 @@ void vd_ctx_destroy(vd_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->side) (void)hipStreamSynchronize(ctx->side);
-  ctx->verts.release(); ctx->sites.release(); ctx->tet_xyz.release();
+  ctx->verts.release(); ctx->sites.release(); ctx->tet_xyz.release(); ctx->tet_prep.release();
   ctx->b_site_begin.release(); ctx->b_voff.release(); ctx->b_cell_base.release(); ctx->b_site_inst.release(); ctx->b_tet_inst.release();
   ctx->b_box.release(); ctx->tets.release(); ctx->offsets.release();
   ctx->shape.release(); ctx->g_count.release(); ctx->g_start.release(); ctx->g_cursor.release();
@@ -489,6 +490,8 @@ int load_mesh(vd_ctx* ctx, const float* verts_xyz, int n_verts, const int32_t* t
     VD_HIP(ctx->tet_xyz.ensure((size_t)3 * std::max(n_tets, 1)), "alloc tet records");
     VD_HIP(hipMemcpy(ctx->tet_xyz.p, rec.data(), sizeof(float) * rec.size(), hipMemcpyHostToDevice),
            "upload tet records");
+    VD_HIP(ctx->tet_prep.ensure((size_t)vd::kTetPrepWords * std::max(n_tets, 1)), "alloc tet records");
+    VD_HIP(vd::tet_prep(ctx->tet_xyz.p, n_tets, ctx->tet_prep.p, ctx->stream), "tet records");
   }
   ctx->n_verts = n_verts;
   ctx->n_tets = n_tets;
@@ -731,6 +734,7 @@ int vd_clip_step(vd_ctx* ctx) {
   a.verts = ctx->verts.p;
   a.tets = ctx->tets.p;
   a.tet_xyz = ctx->tet_xyz.p;
+  a.tet_prep = ctx->tet_prep.p;
   a.tet_knn = ctx->tet_knn.p;
   a.TK = ctx->TK;
   a.n_pairs = n_pairs;
@@ -1368,6 +1372,7 @@ int assemble(vd_ctx* ctx, const JobDev* jobs, int n) {
   VD_HIP(ctx->verts.ensure(std::max(V, 1LL)), "alloc batch");
   VD_HIP(ctx->tets.ensure(std::max(T, 1LL)), "alloc batch");
   VD_HIP(ctx->tet_xyz.ensure(3 * std::max(T, 1LL)), "alloc batch");
+  VD_HIP(ctx->tet_prep.ensure((size_t)vd::kTetPrepWords * std::max(T, 1LL)), "alloc batch");
   VD_HIP(ctx->sites.ensure(std::max(S, 1LL)), "alloc batch");
   for (int i = 0; i < n; ++i) {
     const JobDev& j = jobs[i];
@@ -1389,6 +1394,8 @@ int assemble(vd_ctx* ctx, const JobDev* jobs, int n) {
       ctx->mesh_hi[a] = std::max(ctx->mesh_hi[a], jobs[i].box[3 + a]);
     }
   }
+  // the tets' start records, from the copied corner records
+  VD_HIP(vd::tet_prep(ctx->tet_xyz.p, T, ctx->tet_prep.p, st), "tet records");
   ctx->n_verts = (int)V;
   ctx->n_tets = (int)T;
   ctx->n_sites = (int)S;

@@ -153,9 +153,9 @@ struct vd_server {
   }
 
   // Device bytes of a running job: its own buffers plus its share of the
-  // batch's working memory (batch copies, kNN tables, grouped pairs, retry
-  // list, grid, and the site-major form's volume-pass records: one per ten
-  // pairs).
+  // batch's working memory (batch copies, the tets' start records, kNN
+  // tables, grouped pairs, retry list, grid, and the site-major form's
+  // volume-pass records: one per ten pairs).
   size_t footprint(int nv, int nt, int ns) const {
     const size_t K = (size_t)cfg.site_k, TK = (size_t)cfg.tet_k;
     const size_t own = 16 * (size_t)nv + (16 + 48) * (size_t)nt + 16 * (size_t)ns + 4 * 256;
@@ -165,7 +165,8 @@ struct vd_server {
       rec = (16 + 4 * (size_t)vd::rec_stride(ft)) * TK / 10;
     }
     const size_t work =
-        16 * (size_t)nv + (16 + 48 + 12 * TK + rec + 8) * (size_t)nt + (16 + 4 * K + 64) * (size_t)ns;
+        16 * (size_t)nv + (16 + 48 + 16 * vd::kTetPrepWords + 12 * TK + rec + 8) * (size_t)nt +
+        (16 + 4 * K + 64) * (size_t)ns;
     return own + work;
   }
 
