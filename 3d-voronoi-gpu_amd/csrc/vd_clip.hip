@@ -762,21 +762,11 @@ struct Cell {
   // The same six-tet ring around an apex O, with edge feet taken on the edge
   // line (the library's well-conditioned formulation), integrated in FP64 over
   // planes rebuilt in FP64; v: the tet's corners.  The signed ring is exact for
-  // any apex, so O is the mean of the FP32 vertices.
+  // any apex, so O is the seed (in FP64 a seed outside the polytope costs
+  // nothing measurable).
   template <int ACC>
   __device__ void moments_robust(PairSums<ACC>& S, const P4 v[4], int cell_nv) const {
-    P4 Of = {0.f, 0.f, 0.f, 1.f};
-    for (int t = 0; t < nt; ++t) {
-      const P4 X = vertex(tri(t));
-      Of.x += X.x;
-      Of.y += X.y;
-      Of.z += X.z;
-    }
-    const float inv = 1.f / (float)(nt > 0 ? nt : 1);
-    Of.x *= inv;
-    Of.y *= inv;
-    Of.z *= inv;
-    const D4 O = up4(Of);
+    const D4 O = up4(seed);
     double sv = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;  // 6 x volume, 24 x moments
     for (int t = 0; t < nt; ++t) {
       const uint32_t tt = tri(t);

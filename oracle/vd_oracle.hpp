@@ -566,15 +566,9 @@ inline V4<double> vertex3d(const V4<double>& p1, const V4<double>& p2, const V4<
 }
 template <class R, class A>
 inline bool pair_moments_robust64(const Poly<R>& c, const V4<R> tv[4], int cell_nv, A& vol, A& mx, A& my, A& mz) {
-  // apex: the mean of the FP32 vertices (the signed ring is exact for any apex)
-  V4<R> Of = {R(0), R(0), R(0), R(1)};
-  for (int t = 0; t < c.nt; ++t) {
-    const V4<R> X = c.vertex(t);
-    Of.x += X.x; Of.y += X.y; Of.z += X.z;
-  }
-  const R inv = R(1) / R(c.nt > 0 ? c.nt : 1);
-  Of.x *= inv; Of.y *= inv; Of.z *= inv;
-  const V4<double> O = up64(Of);
+  // apex: the seed (the signed ring is exact for any apex, and in FP64 a
+  // seed outside the polytope costs nothing measurable)
+  const V4<double> O = up64(c.seed);
   double sv = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;  // 6 x volume, 24 x moments
   V4<double> P[6];
   for (int t = 0; t < c.nt; ++t) {
