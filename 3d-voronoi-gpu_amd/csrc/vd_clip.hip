@@ -773,21 +773,36 @@ struct Cell {
       const D4 q[3] = {plane64(tri_a(tt), v, cell_nv), plane64(tri_b(tt), v, cell_nv),
                        plane64(tri_c(tt), v, cell_nv)};
       const D4 X = vertex3d(q[0], q[1], q[2]);
-      D4 P[6];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) P[2 * k] = foot_on_plane_d(O, q[k]);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) P[2 * k + 1] = foot_on_line_d(O, X, cross3d(q[k], q[(k + 1) % 3]));
+      // ring point k: the apex's foot on plane k/2 (k even) or on the line of
+      // planes k/2 and k/2+1 (k odd)
+      auto ring = [&](int k) -> D4 {
+        return (k & 1) ? foot_on_line_d(O, X, cross3d(q[k >> 1], q[((k >> 1) + 1) % 3]))
+                       : foot_on_plane_d(O, q[k >> 1]);
+      };
+      // The six tets (P_k, P_k+1, O, X) share the edge c = O - X:
+      // 6 vol = -det(P_k - X, P_k+1 - X, c) = (P_k - X) . (c x (P_k+1 - X)), and
+      // 24 moment = 6 vol (P_k + P_k+1 + O + X), whose O + X term is summed once.
+      const D4 c = sub3d(O, X);
+      const D4 P0 = ring(0);
+      const D4 a0 = sub3d(P0, X);
+      D4 Pp = P0, ap = a0;
+      double tv = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
-        double vol6;
-        D4 b4;
-        tet_vol6_bary4_d(P[k], P[(k + 1) % 6], O, X, vol6, b4);
-        sx = fma(vol6, b4.x, sx);
-        sy = fma(vol6, b4.y, sy);
-        sz = fma(vol6, b4.z, sz);
-        sv += vol6;
+        const D4 Pn = k < 5 ? ring(k + 1) : P0;
+        const D4 an = k < 5 ? sub3d(Pn, X) : a0;
+        const double v6 = dot3d(ap, cross3d(c, an));
+        mx = fma(v6, Pp.x + Pn.x, mx);
+        my = fma(v6, Pp.y + Pn.y, my);
+        mz = fma(v6, Pp.z + Pn.z, mz);
+        tv += v6;
+        Pp = Pn;
+        ap = an;
       }
+      sx = fma(O.x + X.x, tv, sx + mx);
+      sy = fma(O.y + X.y, tv, sy + my);
+      sz = fma(O.z + X.z, tv, sz + mz);
+      sv += tv;
     }
     S.set(sv / 6.0, sx / 24.0, sy / 24.0, sz / 24.0);
   }

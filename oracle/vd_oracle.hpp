@@ -570,23 +570,31 @@ inline bool pair_moments_robust64(const Poly<R>& c, const V4<R> tv[4], int cell_
   // seed outside the polytope costs nothing measurable)
   const V4<double> O = up64(c.seed);
   double sv = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;  // 6 x volume, 24 x moments
-  V4<double> P[6];
+  V4<double> P[6], a[6];
   for (int t = 0; t < c.nt; ++t) {
     const V4<double> q[3] = {plane64(c, c.tr[t][0], tv, cell_nv), plane64(c, c.tr[t][1], tv, cell_nv),
                              plane64(c, c.tr[t][2], tv, cell_nv)};
     const V4<double> X = vertex3d(q[0], q[1], q[2]);
     for (int i = 0; i < 3; ++i) P[2 * i] = foot_on_plane(O, q[i]);
     for (int i = 0; i < 3; ++i) P[2 * i + 1] = foot_on_line(O, X, cross3(q[i], q[(i + 1) % 3]));
+    // the six tets (P_i, P_i+1, O, X) share the edge cc = O - X:
+    // 6 vol = -det(P_i - X, P_i+1 - X, cc) = (P_i - X) . (cc x (P_i+1 - X)), and
+    // 24 moment = 6 vol (P_i + P_i+1 + O + X), whose O + X term is summed once
+    const V4<double> cc = sub3(O, X);
+    for (int i = 0; i < 6; ++i) a[i] = sub3(P[i], X);
+    double tvol = 0.0, mx6 = 0.0, my6 = 0.0, mz6 = 0.0;
     for (int i = 0; i < 6; ++i) {
-      const V4<double>& Ap = P[i];
-      const V4<double>& Bp = P[(i + 1) % 6];
-      const V4<double> a = sub3(Ap, X), b = sub3(Bp, X), cc = sub3(O, X);
-      const double vol6 = -det3(a.x, a.y, a.z, b.x, b.y, b.z, cc.x, cc.y, cc.z);
-      sx = std::fma(vol6, ((Ap.x + Bp.x) + O.x) + X.x, sx);
-      sy = std::fma(vol6, ((Ap.y + Bp.y) + O.y) + X.y, sy);
-      sz = std::fma(vol6, ((Ap.z + Bp.z) + O.z) + X.z, sz);
-      sv += vol6;
+      const int j = (i + 1) % 6;
+      const double vol6 = dot3(a[i], cross3(cc, a[j]));
+      mx6 = std::fma(vol6, P[i].x + P[j].x, mx6);
+      my6 = std::fma(vol6, P[i].y + P[j].y, my6);
+      mz6 = std::fma(vol6, P[i].z + P[j].z, mz6);
+      tvol += vol6;
     }
+    sx = std::fma(O.x + X.x, tvol, sx + mx6);
+    sy = std::fma(O.y + X.y, tvol, sy + my6);
+    sz = std::fma(O.z + X.z, tvol, sz + mz6);
+    sv += tvol;
   }
   vol = (A)(sv / 6.0); mx = (A)(sx / 24.0); my = (A)(sy / 24.0); mz = (A)(sz / 24.0);
   if (c.ev) { c.ev->acc_tris += c.nt; c.ev->acc_pairs += 1; }
