@@ -61,6 +61,9 @@ int site_depth_of(const vd_config& c, int K) { return std::min(c.site_depth > 0 
 // re-clip from scratch cost more than the deeper tables' lost occupancy (one
 // wave per CU fewer at the same vertex cap).
 constexpr int kFreshSiteDepth = 64;
+// kNN steps after the caller sets sites that do without the previous rows'
+// bounds (see vd_knn_step)
+constexpr int kHintAfter = 2;
 
 int tet_k_heuristic(int n_site, int n_tet, int preferred) {
   // voronoi.cu:936
@@ -156,6 +159,9 @@ struct vd_ctx {
   // sites set by the caller and not clipped since (the next site-major
   // clipping at the automatic depth uses kFreshSiteDepth)
   bool fresh_sites = false;
+  // kNN steps since the caller set sites: the first two do without bounds
+  // from the previous rows (kHintAfter)
+  int knn_since_set = 0;
   // site-major pairs
   DevBuf<int> p_count, p_cursor;
   DevBuf<unsigned> p_start, p_ids;
@@ -586,6 +592,7 @@ int vd_set_batch(vd_ctx* ctx, int n_inst, const int32_t* n_verts, const int32_t*
   if (rc != VD_OK) return rc;
   ctx->have_knn = ctx->have_clip = false;
   ctx->fresh_sites = true;
+  ctx->knn_since_set = 0;
   return VD_OK;
 }
 
@@ -612,6 +619,7 @@ int vd_set_sites(vd_ctx* ctx, const float* sites_xyz, int n_sites) {
   ctx->n_sites = n_sites;
   ctx->have_knn = ctx->have_clip = false;
   ctx->fresh_sites = true;
+  ctx->knn_since_set = 0;
   return VD_OK;
 }
 
@@ -675,7 +683,18 @@ int vd_knn_step(vd_ctx* ctx) {
   const bool tets = ctx->n_tets > 0 && ctx->TK > 0;
   const bool fork = ctx->K > 0 && tets;
   // the rows of the last kNN step bound this one's queries (same points, moved)
-  const float4* hint = ctx->rows_valid && ctx->rows_K == ctx->K && ctx->rows_TK == ctx->TK ? ctx->sites.p : nullptr;
+  // The previous rows bound each query (any k of its instance's points do)
+  // but cost a gather per row; when the sites have moved far since those rows
+  // were made the bound is loose and costs more than it saves.  Sites set by
+  // the caller (random samples, typically) and their first Lloyd update move
+  // far: kNN iteration 1 2.31-2.50 ms with the bounds against 2.17-2.21
+  // without, iteration 2 2.26-2.30 against 2.15-2.20, iteration 3 2.08-2.11
+  // either way (1M tets, 100k uniform sites).
+  const float4* hint = ctx->rows_valid && ctx->rows_K == ctx->K && ctx->rows_TK == ctx->TK &&
+                               ctx->knn_since_set >= kHintAfter
+                           ? ctx->sites.p
+                           : nullptr;
+  ++ctx->knn_since_set;
   if (fork) {
     VD_HIP(hipEventRecord(ctx->fork_ev, ctx->stream), "event");
     VD_HIP(hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0), "stream wait");
@@ -971,6 +990,9 @@ int vd_lloyd_graph(vd_ctx* ctx, int n_iter) {
     if (rc != VD_OK) return rc;
     if (--n_iter == 0) return VD_OK;
     VD_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+    // the captured iteration stands for every later one: it takes the
+    // previous rows' bounds whenever the rows are valid
+    ctx->knn_since_set = std::max(ctx->knn_since_set, kHintAfter);
     VD_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     ctx->capturing = true;
     rc = vd_knn_step(ctx);
