@@ -1959,18 +1959,28 @@ __device__ int hash_insert(int* keys, int site) {
 
 __global__ __launch_bounds__(256) void k_pairs_count(const int32_t* tet_knn, long long n_pairs, int n_sites,
                                                       int* count) {
+  constexpr int kPerThread = kGroupPairs / 256;
+  static_assert(kGroupPairs % 256 == 0, "count group");
   __shared__ int keys[kHashSlots];
   __shared__ int vals[kHashSlots];
+  // this thread's pairs' sites first, all their loads in flight at once (a
+  // load per trip behind the previous trip's table atomics waited out the
+  // full memory latency every trip)
+  const long long base = (long long)blockIdx.x * kGroupPairs;
+  int sv[kPerThread];
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const long long p = base + k * 256 + threadIdx.x;
+    sv[k] = p < n_pairs ? tet_knn[p] : -1;
+  }
   for (int i = threadIdx.x; i < kHashSlots; i += blockDim.x) {
     keys[i] = kEmptySlot;
     vals[i] = 0;
   }
   __syncthreads();
-  const long long base = (long long)blockIdx.x * kGroupPairs;
-  for (int j = threadIdx.x; j < kGroupPairs; j += blockDim.x) {
-    const long long p = base + j;
-    if (p >= n_pairs) break;
-    const int s = tet_knn[p];
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const int s = sv[k];
     if ((unsigned)s >= (unsigned)n_sites) continue;
     const int h = hash_insert(keys, s);
     if (h >= 0)
@@ -2008,20 +2018,26 @@ __global__ __launch_bounds__(kScatterThreads) void k_pairs_scatter(const int32_t
   __shared__ uint16_t stage_slot[kScatterPairs];  // their slots
   unsigned* wsum = stage;  // the waves' count sums while the slots are scanned
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const long long base = (long long)blockIdx.x * kScatterPairs;
+  // this thread's pairs' sites first, all their loads in flight at once
+  int sv[kScatterPerThread];
+#pragma unroll
+  for (int k = 0; k < kScatterPerThread; ++k) {
+    const long long p = base + k * kScatterThreads + tid;
+    sv[k] = p < n_pairs ? tet_knn[p] : -1;
+  }
   for (int i = tid; i < kHashSlots; i += kScatterThreads) {
     keys[i] = kEmptySlot;
     cnt[i] = 0;
   }
   __syncthreads();
-  const long long base = (long long)blockIdx.x * kScatterPairs;
   // slot (high 16 bits) and rank within it (low 16) of this thread's pairs
   uint32_t code[kScatterPerThread];
 #pragma unroll
   for (int k = 0; k < kScatterPerThread; ++k) {
     const long long p = base + k * kScatterThreads + tid;
     code[k] = ~0u;
-    if (p >= n_pairs) continue;
-    const int s = tet_knn[p];
+    const int s = sv[k];
     if ((unsigned)s >= (unsigned)n_sites) continue;
     const int h = hash_insert(keys, s);
     if (h >= 0) {
