@@ -1294,23 +1294,29 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
         c.status = kSecurityRadiusNotReached;
         out = 1;
       } else {
-        for (int it = 0; it < kTestsPerTrip && out == 0; ++it) {
-          // |z - s|^2 from the bisector's normal s - z, the same bits as from
-          // z - s; NaN marks an out-of-range neighbour
-          const P4 e = ld4(c.ps.bis, pv);
-          const float d2z = dot3(e, e);
-          if (d2z != d2z) {
-            c.status = kInconsistentBoundary;
-            out = 2;
-            break;
+        // (a wave-uniform loop with the lanes that stop predicated off: the
+        // same tests as a per-lane loop with breaks, and cheaper control flow)
+        for (int it = 0; it < kTestsPerTrip; ++it) {
+          const bool testing = out == 0 && !cut;
+          if (__ballot(testing) == 0) break;
+          if (testing) {
+            // |z - s|^2 from the bisector's normal s - z, the same bits as
+            // from z - s; NaN marks an out-of-range neighbour
+            const P4 e = ld4(c.ps.bis, pv);
+            const float d2z = dot3(e, e);
+            if (d2z != d2z) {
+              c.status = kInconsistentBoundary;
+              out = 2;
+            } else {
+              const int r = c.clip_test(e, d2z, conf);
+              if (r == 2) {
+                cut = true;
+                cut_d2z = d2z;
+              } else {
+                out = step_end(r, d2z);
+              }
+            }
           }
-          const int r = c.clip_test(e, d2z, conf);
-          if (r == 2) {
-            cut = true;
-            cut_d2z = d2z;
-            break;
-          }
-          out = step_end(r, d2z);
         }
       }
       if (cut) out = step_end(c.clip_cut(4 + pv, conf, cut_d2z), cut_d2z);
