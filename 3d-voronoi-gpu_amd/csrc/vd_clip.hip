@@ -281,36 +281,11 @@ struct Cell {
     ps.put(3, tri2plane(v[0], v[1], v[2]));
   }
 
-  // Tet start: the faces and the tet's 4 vertices (voronoi.cu:163-201).
-  __device__ void init_tet(const P4& s, const P4 v[4]) {
-    seed = s;
-    status = kSuccess;
-    keep_srnr = false;
-    put_tet_faces(v);
-    nv = 4;
-    float m = 0.f;
-    m = fmaxf(put_tri(0, pack3(1, 3, 2)), m);
-    m = fmaxf(put_tri(1, pack3(0, 1, 2)), m);
-    m = fmaxf(put_tri(2, pack3(0, 3, 1)), m);
-    m = fmaxf(put_tri(3, pack3(0, 2, 3)), m);
-    vmax = m;
-    nt = 4;
-    first = kNoPlane;
-    far = 0xFu;
-    last_d2z = 0.f;
-    if constexpr (TS::kCached) {
-      blo[0] = fminf(fminf(v[0].x, v[1].x), fminf(v[2].x, v[3].x)) - s.x;
-      blo[1] = fminf(fminf(v[0].y, v[1].y), fminf(v[2].y, v[3].y)) - s.y;
-      blo[2] = fminf(fminf(v[0].z, v[1].z), fminf(v[2].z, v[3].z)) - s.z;
-      bhi[0] = fmaxf(fmaxf(v[0].x, v[1].x), fmaxf(v[2].x, v[3].x)) - s.x;
-      bhi[1] = fmaxf(fmaxf(v[0].y, v[1].y), fmaxf(v[2].y, v[3].y)) - s.y;
-      bhi[2] = fmaxf(fmaxf(v[0].z, v[1].z), fmaxf(v[2].z, v[3].z)) - s.z;
-    }
-  }
-
-  // The same start from a tet's precomputed record (tet_prep: the four face
-  // planes, then the vertices of triangles 0..3 as x0 y0 z0 x1 | y1 z1 x2 y2 |
-  // z2 x3 y3 z3), which holds what put_tet_faces and put_tri compute.
+  // Tet start (voronoi.cu:163-201): the faces cbd, acd, bad, abc as plane
+  // ids 0..3 and the tet's 4 vertices as triangles (1,3,2), (0,1,2), (0,3,1),
+  // (0,2,3), from the tet's precomputed record (tet_prep: the four face planes,
+  // then the vertices of triangles 0..3 as x0 y0 z0 x1 | y1 z1 x2 y2 |
+  // z2 x3 y3 z3), which holds what put_tet_faces and put_tri would compute.
   __device__ void init_tet_prep(const P4& s, const float4* rec) {
     seed = s;
     status = kSuccess;

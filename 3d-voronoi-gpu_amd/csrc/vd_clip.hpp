@@ -13,8 +13,9 @@ struct ClipArgs {
   const float4* verts;       // (x, y, z, _)
   const int4* tets;          // 4 vertex ids, positively oriented
   const float4* tet_xyz;     // site-major form: per tet its corners, 3 float4 (12 floats)
-  const float4* tet_prep;    // optional (tet_prep()): per tet its 4 face planes and the 4
-                             // vertices they make, as the tet start computes them, 7 float4
+  const float4* tet_prep;    // tet-major and site-major forms (tet_prep()): per tet its 4 face
+                             // planes and the 4 vertices they make, 7 float4; a pair's start
+                             // loads them
   const int32_t* tet_knn;    // n_tets x TK candidate sites
   int TK;
   long long n_pairs;         // n_tets * TK
