@@ -542,6 +542,78 @@ struct Cell {
     return 2;
   }
 
+  // The next bisectors e[0..n) of the neighbour list tested against the same
+  // polytope, as clip_test tests them one at a time (filter off): no vertex is
+  // removed before the first bisector that conflicts, so the tests share one
+  // pass over the far vertices (with the smallest near ball of the batch) and
+  // one pass over the candidates, in which a vertex's minors serve every
+  // bisector's determinant.  conf[k]: the vertices in conflict with bisector
+  // k, 0 where its box test holds (k >= n: unused).  Each determinant is
+  // clip_test's, bit for bit, and a candidate of the batch that is not one of
+  // bisector k's own lies inside k's near ball, where its determinant is
+  // negative, so the first k with conf[k] != 0 and conf[k] itself are what the
+  // tests one at a time find.  The caller checks the vertex cap first.
+  template <int N>
+  __device__ __forceinline__ void test_batch(const P4 (&e)[N], const float (&d2z)[N], int n, uint32_t (&conf)[N]) {
+    bool skip[N];
+    float dmin = INFINITY;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const float px = -fminf(blo[0] * e[k].x, bhi[0] * e[k].x);
+      const float py = -fminf(blo[1] * e[k].y, bhi[1] * e[k].y);
+      const float pz = -fminf(blo[2] * e[k].z, bhi[2] * e[k].z);
+      skip[k] = k >= n || !(d2z[k] == d2z[k]) || px + py + pz < 0.45f * d2z[k];
+      dmin = skip[k] ? dmin : fminf(dmin, d2z[k]);
+      conf[k] = 0u;
+    }
+    if (dmin == INFINITY) return;
+    if (!(dmin >= last_d2z)) far = (1u << nt) - 1u;
+    last_d2z = dmin;
+    const float near2 = 0.2025f * dmin;
+    uint32_t cand = 0;
+    for (uint32_t m = far; m;) {
+      int t[kFarUnroll];
+      bool has[kFarUnroll];
+      float d[kFarUnroll];
+#pragma unroll
+      for (int j = 0; j < kFarUnroll; ++j) {
+        has[j] = m != 0;
+        t[j] = has[j] ? __builtin_ctz(m) : 0;
+        m &= m - 1;
+      }
+#pragma unroll
+      for (int j = 0; j < kFarUnroll; ++j) d[j] = ts.d2(t[j]);
+#pragma unroll
+      for (int j = 0; j < kFarUnroll; ++j) {
+        const uint32_t bit = has[j] ? 1u << t[j] : 0u;
+        const uint32_t keep = d[j] < near2 ? 0u : bit;
+        far &= ~(bit & ~keep);
+        cand |= keep;
+      }
+    }
+    for (uint32_t m = cand; m;) {
+      int t[kCandUnroll];
+      bool has[kCandUnroll];
+      uint32_t tt[kCandUnroll];
+#pragma unroll
+      for (int j = 0; j < kCandUnroll; ++j) {
+        has[j] = m != 0;
+        t[j] = has[j] ? __builtin_ctz(m) : 0;
+        m &= m - 1;
+      }
+#pragma unroll
+      for (int j = 0; j < kCandUnroll; ++j) tt[j] = tri(t[j]);
+#pragma unroll
+      for (int j = 0; j < kCandUnroll; ++j) {
+        const Minors mn = vertex_minors(ps.get(tri_a(tt[j])), ps.get(tri_b(tt[j])), ps.get(tri_c(tt[j])));
+#pragma unroll
+        for (int k = 0; k < N; ++k) conf[k] |= has[j] && conflict_det(mn, e[k]) > 0.f ? 1u << t[j] : 0u;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) conf[k] = skip[k] ? 0u : conf[k];
+  }
+
   // clip_cut: removal of the flagged vertices, the cavity walk and the fan of
   // new vertices around plane `pid`; returns as clip().
   __device__ __forceinline__ int clip_cut(int pid, uint32_t conf, float d2z) {
@@ -1268,6 +1340,42 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
       if (K == 0) {
         c.status = kSecurityRadiusNotReached;
         out = 1;
+      } else if (!a.filter) {
+        // the trip's bisectors tested as one batch (test_batch), then their
+        // outcomes taken in list order as the tests one at a time take them
+        P4 e[kTestsPerTrip];
+        float d2z[kTestsPerTrip];
+        uint32_t cf[kTestsPerTrip];
+#pragma unroll
+        for (int k = 0; k < kTestsPerTrip; ++k) {
+          // |z - s|^2 from the bisector's normal s - z, the same bits as
+          // from z - s; NaN marks an out-of-range neighbour
+          e[k] = ld4(c.ps.bis, min(pv + k, K - 1));
+          d2z[k] = dot3(e[k], e[k]);
+          cf[k] = 0u;
+        }
+        const bool capped = c.nv >= c.pcap;
+        if (!capped) c.test_batch(e, d2z, min(kTestsPerTrip, K - pv), cf);
+#pragma unroll
+        for (int k = 0; k < kTestsPerTrip; ++k) {
+          if (out != 0 || cut) continue;
+          const float d = d2z[k];
+          if (d != d) {
+            c.status = kInconsistentBoundary;
+            out = 2;
+          } else if (capped) {
+            c.status = kVertexOverflow;
+            out = step_end(-1, d);
+          } else if (cf[k] != 0u) {
+            cut = true;
+            cut_d2z = d;
+            conf = cf[k];
+            ++c.nv;
+            c.nr = 0;
+          } else {
+            out = step_end(d > 4.f * c.vmax ? 1 : 0, d);
+          }
+        }
       } else {
         // (a wave-uniform loop with the lanes that stop predicated off: the
         // same tests as a per-lane loop with breaks, and cheaper control flow)
