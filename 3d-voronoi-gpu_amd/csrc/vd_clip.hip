@@ -41,7 +41,7 @@ __device__ __forceinline__ int tri_at(uint32_t t, int k) { return (int)((t >> (8
 #define VD_FAR_UNROLL 4
 #endif
 #ifndef VD_CAND_UNROLL
-#define VD_CAND_UNROLL 4
+#define VD_CAND_UNROLL 1
 #endif
 constexpr int kFarUnroll = VD_FAR_UNROLL, kCandUnroll = VD_CAND_UNROLL;
 
@@ -51,6 +51,11 @@ constexpr int kFarUnroll = VD_FAR_UNROLL, kCandUnroll = VD_CAND_UNROLL;
 #define VD_TESTS_PER_TRIP 3
 #endif
 constexpr int kTestsPerTrip = VD_TESTS_PER_TRIP;
+// Bisectors the site-major kernel tests per trip as one batch (test_batch).
+#ifndef VD_BATCH_TESTS
+#define VD_BATCH_TESTS 6
+#endif
+constexpr int kBatchTests = VD_BATCH_TESTS;
 
 // Storage policies --------------------------------------------------------
 //
@@ -1343,11 +1348,11 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
       } else if (!a.filter) {
         // the trip's bisectors tested as one batch (test_batch), then their
         // outcomes taken in list order as the tests one at a time take them
-        P4 e[kTestsPerTrip];
-        float d2z[kTestsPerTrip];
-        uint32_t cf[kTestsPerTrip];
+        P4 e[kBatchTests];
+        float d2z[kBatchTests];
+        uint32_t cf[kBatchTests];
 #pragma unroll
-        for (int k = 0; k < kTestsPerTrip; ++k) {
+        for (int k = 0; k < kBatchTests; ++k) {
           // |z - s|^2 from the bisector's normal s - z, the same bits as
           // from z - s; NaN marks an out-of-range neighbour
           e[k] = ld4(c.ps.bis, min(pv + k, K - 1));
@@ -1355,9 +1360,9 @@ __global__ __launch_bounds__(64) void k_clip_site(ClipArgs a) {
           cf[k] = 0u;
         }
         const bool capped = c.nv >= c.pcap;
-        if (!capped) c.test_batch(e, d2z, min(kTestsPerTrip, K - pv), cf);
+        if (!capped) c.test_batch(e, d2z, min(kBatchTests, K - pv), cf);
 #pragma unroll
-        for (int k = 0; k < kTestsPerTrip; ++k) {
+        for (int k = 0; k < kBatchTests; ++k) {
           if (out != 0 || cut) continue;
           const float d = d2z[k];
           if (d != d) {
