@@ -820,8 +820,12 @@ struct Cell {
   __device__ void moments_robust(PairSums<ACC>& S, const P4 v[4], int cell_nv) const {
     const D4 O = up4(seed);
     double sv = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;  // 6 x volume, 24 x moments
+    // the next vertex's triangle is read one vertex ahead (in the volume pass
+    // it comes from the record in global memory)
+    uint32_t tt_next = nt > 0 ? tri(0) : 0u;
     for (int t = 0; t < nt; ++t) {
-      const uint32_t tt = tri(t);
+      const uint32_t tt = tt_next;
+      tt_next = tri(min(t + 1, nt - 1));
       const D4 q[3] = {plane64(tri_a(tt), v, cell_nv), plane64(tri_b(tt), v, cell_nv),
                        plane64(tri_c(tt), v, cell_nv)};
       const D4 X = vertex3d(q[0], q[1], q[2]);
@@ -1730,22 +1734,44 @@ __global__ __launch_bounds__(kMomentsBlock) void k_site_moments(ClipArgs a) {
     const bool rec = m.x != ~0u;  // a reserved slot left unused, or past the end
     const unsigned pair = m.x;
     const int s = rec ? (int)m.y : 0;
-    // the bisector tables of the wave's first kMomentsTables seeds
+    // the bisector tables of the wave's first kMomentsTables seeds: the
+    // seeds first, then every table entry's loads at once
     int slot = -1;
     {
+      int kst[kMomentsTables];
+      int ntab = 0;
       unsigned long long todo = __ballot(rec);
-      for (int t = 0; t < kMomentsTables && todo; ++t) {
+#pragma unroll
+      for (int t = 0; t < kMomentsTables; ++t) {
+        kst[t] = 0;
+        if (!todo) continue;
         const int leader = __builtin_ctzll(todo);
         const int ks = __shfl(s, leader);
         const bool mine = rec && s == ks;
         todo &= ~__ballot(mine);
         if (mine) slot = t;
-        if (lane < depth) {
-          const int z = a.site_knn[(size_t)ks * a.K + lane];
-          const P4 p = (unsigned)z >= (unsigned)a.n_sites ? P4{0.f, 0.f, 0.f, 0.f}
-                                                          : bisector(site_at(a, ks), site_at(a, z));
-          tbl[t][lane] = make_float4(p.x, p.y, p.z, p.w);
-        }
+        kst[t] = ks;
+        ntab = t + 1;
+      }
+      constexpr int kRounds = (kMomentsTables * kMomentsDepth + 63) / 64;
+      int z[kRounds], ks[kRounds], tv[kRounds];
+#pragma unroll
+      for (int r = 0; r < kRounds; ++r) {
+        const int ent = r * 64 + lane, t = depth > 0 ? ent / depth : kMomentsTables;
+        tv[r] = t < ntab ? ent : -1;
+        int k = kst[0];
+#pragma unroll
+        for (int j = 1; j < kMomentsTables; ++j) k = t == j ? kst[j] : k;
+        ks[r] = k;
+        z[r] = tv[r] >= 0 ? a.site_knn[(size_t)k * a.K + (ent - t * depth)] : -1;
+      }
+#pragma unroll
+      for (int r = 0; r < kRounds; ++r) {
+        if (tv[r] < 0) continue;
+        const int t = tv[r] / depth;
+        const P4 p = (unsigned)z[r] >= (unsigned)a.n_sites ? P4{0.f, 0.f, 0.f, 0.f}
+                                                           : bisector(site_at(a, ks[r]), site_at(a, z[r]));
+        tbl[t][tv[r] - t * depth] = make_float4(p.x, p.y, p.z, p.w);
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
