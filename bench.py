@@ -334,7 +334,7 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=0, help="also time a device copy of this size (MiB)")
     ap.add_argument("--batch-steps", type=int, default=10, help="graph replays of the 256-instance batch (0 skips)")
     ap.add_argument("--serve-jobs", type=int, default=120, help="jobs of the served workload (0 skips)")
-    ap.add_argument("--serve-rate", type=float, nargs="+", default=[8.0], help="job arrivals per second")
+    ap.add_argument("--serve-rate", type=float, nargs="+", default=[8.0, 16.0], help="job arrivals per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
