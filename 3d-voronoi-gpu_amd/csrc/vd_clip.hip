@@ -176,6 +176,7 @@ struct TriPlain {
     tr[a * 64 + lane] = y;
     tr[b * 64 + lane] = x;
   }
+  __device__ __forceinline__ void copy(int dst, int src) { tr[dst * 64 + lane] = tr[src * 64 + lane]; }
 };
 struct TriCached {
   static constexpr bool kCached = true;
@@ -189,6 +190,14 @@ struct TriCached {
     tr[a * 64 + lane] = y;
     tr[b * 64 + lane] = x;
   }
+  // swap, returning the triangle that moves to slot b
+  __device__ __forceinline__ uint32_t swap_get(int a, int b) {
+    const uint2 x = tr[a * 64 + lane], y = tr[b * 64 + lane];
+    tr[a * 64 + lane] = y;
+    tr[b * 64 + lane] = x;
+    return x.x;
+  }
+  __device__ __forceinline__ void copy(int dst, int src) { tr[dst * 64 + lane] = tr[src * 64 + lane]; }
 };
 
 // TriRecord: the triangles of a volume-pass record (read only).
@@ -376,9 +385,11 @@ struct Cell {
 
   // Boundary loop of the removed triangles tr[nt..nt+nr) (voronoi.cu:391-441):
   // removed triangles are merged one at a time into a circular list nx[] of
-  // planes; a triangle whose merge would pinch the loop is deferred.
-  __device__ void cavity_boundary() {
-    for (int r = nt; r < nt + nr; ++r) {
+  // planes; a triangle whose merge would pinch the loop is deferred.  The
+  // links of the removed triangles' planes start cleared (`cleared`: the
+  // caller cleared them as it removed the triangles).
+  __device__ void cavity_boundary(bool cleared = false) {
+    for (int r = nt; r < nt + nr && !cleared; ++r) {
       const uint32_t t = tri(r);
       set_nxt(tri_a(t), kNoPlane);
       set_nxt(tri_b(t), kNoPlane);
@@ -421,7 +432,9 @@ struct Cell {
       set_nxt(e0, op2 && op0 ? kNoPlane : (op0 ? n0 : e1));
       set_nxt(e1, op0 && op1 ? kNoPlane : (op1 ? n1 : e2));
       set_nxt(e2, op1 && op2 ? kNoPlane : (op2 ? n2 : e0));
-      ts.swap(cur, nt + nr - 1);
+      // the merged triangle is not needed again: the last unmerged one takes
+      // its slot
+      ts.copy(cur, nt + nr - 1);
       cur = nt;
       --nr;
     }
@@ -631,10 +644,15 @@ struct Cell {
       const int i = __builtin_ctz(conf);
       --nt;
       ++nr;
-      // slot i takes slot nt's triangle and flags (a no-op swap when i == nt)
+      // slot i takes slot nt's triangle and flags (a no-op swap when i == nt);
+      // the removed triangle's planes get their links cleared for the
+      // cavity walk
       const uint32_t bi = 1u << i, bn = 1u << nt;
       const bool mv = i != nt;
-      ts.swap(i, nt);
+      const uint32_t rt = ts.swap_get(i, nt);
+      set_nxt(tri_a(rt), kNoPlane);
+      set_nxt(tri_b(rt), kNoPlane);
+      set_nxt(tri_c(rt), kNoPlane);
       conf = (conf & ~(bi | bn)) | (mv && (conf & bn) ? bi : 0u);
       far = (far & ~(bi | bn)) | (mv && (far & bn) ? bi : 0u);
     }
@@ -646,7 +664,7 @@ struct Cell {
     // security test: every vertex closer than half of |z - s|)
     float m = 0.f;
     for (int t = 0; t < nt; ++t) m = fmaxf(ts.d2(t), m);
-    cavity_boundary();
+    cavity_boundary(true);
     if (status != kSuccess) return -1;
     if (first == kNoPlane) {
       vmax = m;
