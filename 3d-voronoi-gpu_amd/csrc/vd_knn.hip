@@ -517,6 +517,112 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
   // q0 + j once it is known (-2 before; -1: that row gives none)
   const bool use_hint = a.hint_pts && a.out_pitch <= 0 && (a.mode == 0 || a.mode == 1);
   float run_hb = -2.f;
+  // The run's point list: every point of the instance within sqrt(R2) of c,
+  // with its id in w; returns their number, or -1 when they do not fit in
+  // kUnionCap or the offset table does not reach that far (beyond it the
+  // walk scans the whole instance: no list).
+  auto gather = [&](const float4& c, float R2, const GridShape& g, int cbase) -> int {
+    const int cc[3] = {cell_coord(c.x, g.lo[0], g.inv_h, g.dim[0]), cell_coord(c.y, g.lo[1], g.inv_h, g.dim[1]),
+                       cell_coord(c.z, g.lo[2], g.inv_h, g.dim[2])};
+    int n = 0;
+    bool fits = true, covered = false;
+    for (int ob = 0; ob < a.n_offsets && fits; ob += 64) {
+      const float lbd = fmaxf(0.f, sqrtf((float)a.offsets[ob].w) * g.h - 2.f * g.slack);
+      if (lbd * lbd * (1.f - 1e-5f) > R2) {
+        covered = true;  // offsets are sorted by lb: the rest are beyond R too
+        break;
+      }
+      int cnt = 0, st = 0;
+      const int oi = ob + lane;
+      if (oi < a.n_offsets) {
+        const int4 o = a.offsets[oi];
+        const int ox = cc[0] + o.x, oy = cc[1] + o.y, oz = cc[2] + o.z;
+        if (ox >= 0 && ox < g.dim[0] && oy >= 0 && oy < g.dim[1] && oz >= 0 && oz < g.dim[2] &&
+            !(cell_bound(g, c, ox, oy, oz) > R2)) {
+          const int cell = cbase + ox + g.dim[0] * (oy + g.dim[1] * oz);
+          st = a.start[cell];
+          cnt = a.start[cell + 1] - st;
+        }
+      }
+      const int incl = wave_incl_sum(cnt);
+      const int excl = incl - cnt;
+      const int total = __shfl(incl, 63);
+      int carry = 0;
+      for (int f0 = 0; f0 < total && fits; f0 += 64) {
+        const int f = f0 + lane;
+        const int own = batch_owner(mark, lane, cnt, excl, f0, carry);
+        const int o_st = __shfl(st, own), o_ex = __shfl(excl, own);
+        bool keep = false;
+        float4 p = {0.f, 0.f, 0.f, 0.f};
+        if (f < total) {
+          const int idx = o_st + (f - o_ex);
+          p = a.sorted_pts[idx];
+          p.w = __uint_as_float((uint32_t)a.sorted_ids[idx]);
+          keep = !(dist2(p, c) > R2);
+        }
+        const uint64_t m = __ballot(keep);
+        const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+        const int add = __popcll(m);
+        if (n + add > kUnionCap) {
+          fits = false;
+        } else {
+          if (keep) uni[n + below] = p;
+          n += add;
+        }
+      }
+    }
+    if (!(fits && covered)) return -1;
+    wave_sync();
+    return n;
+  };
+  // With previous results (hint_pts) every query of the run has a bound on its
+  // k-th distance before any walk: the run's list is then gathered first,
+  // within max_j(|q_j - c| + r_j) of the centre c of the run's queries (r_j:
+  // query j's bound), and every query of the run, the first one included,
+  // scans it instead of walking the grid, with the walk's keys and bounds.
+  if (use_hint) {
+    const int n_run = min(q0 + kQueryRun, nq) - q0;
+    float4 pj = {0.f, 0.f, 0.f, 0.f};
+    int sj = -1, rj = 0, ij = 0;
+    const bool in = lane < n_run;
+    if (in) query_point(a, q0 + lane, pj, sj, rj, ij);
+    const int inst0 = __shfl(ij, 0);
+    if (n_run > 0 && __ballot(in && ij != inst0) == 0) {
+      const int lo = a.pt_begin ? a.pt_begin[inst0] : 0, hi = a.pt_begin ? a.pt_begin[inst0 + 1] : a.n_ref;
+      for (int j = 0; j < n_run; ++j) {
+        const float4 qj4 = make_float4(__shfl(pj.x, j), __shfl(pj.y, j), __shfl(pj.z, j), 0.f);
+        const float hb = hint_bound<M>(a, __shfl(rj, j), qj4, __shfl(sj, j), lo, hi, lane);
+        if (lane == j) run_hb = hb;
+      }
+      if (__ballot(in && !(run_hb >= 0.f)) == 0) {
+        float cx = in ? pj.x : 0.f, cy = in ? pj.y : 0.f, cz = in ? pj.z : 0.f;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          cx += __shfl_xor(cx, o);
+          cy += __shfl_xor(cy, o);
+          cz += __shfl_xor(cz, o);
+        }
+        const float inv = 1.f / (float)n_run;
+        const float4 c = make_float4(cx * inv, cy * inv, cz * inv, 0.f);
+        float need = 0.f;
+        if (in) {
+          const float ax = pj.x - c.x, ay = pj.y - c.y, az = pj.z - c.z;
+          need = sqrtf(fmaf(az, az, fmaf(ay, ay, ax * ax))) + sqrtf(run_hb);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) need = fmaxf(need, __shfl_xor(need, o));
+        const GridShape g = a.grid[inst0];
+        // rounding margins of the FP32 distances on either side
+        const float R = need * (1.f + 1e-4f) + 1e-3f * g.h;
+        const int n = gather(c, R * R, g, a.cell_base ? a.cell_base[inst0] : 0);
+        if (n >= 0) {
+          n_uni = n;
+          uni_inst = inst0;
+          r0 = INFINITY;  // the queries' own bounds hold
+        }
+      }
+    }
+  }
   for (int q = q0; q < q0 + kQueryRun && q < nq; ++q) {  // wave-uniform
 
   float4 qp;
@@ -687,7 +793,7 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
   prev_inst = inst;
 
   const int q_end = min(q0 + kQueryRun, nq);
-  if (q == q0 && prev_r >= 0.f && q + 1 < q_end) {
+  if (q == q0 && n_uni < 0 && prev_r >= 0.f && q + 1 < q_end) {
     // the run's other queries: their points (lane j takes query q0 + j)
     float4 pj = qp;
     int sj = self, rj = out_row, ij = inst;
@@ -729,58 +835,8 @@ __global__ __launch_bounds__(64 * kQueryWaves) void k_knn_query(QueryArgs a) {
       // rounding margins of the FP32 distances on either side
       const float R = rj_need * (1.f + 1e-4f) + 1e-3f * g.h;
       const float R2 = R * R;
-      const int cc[3] = {cell_coord(c.x, g.lo[0], g.inv_h, g.dim[0]), cell_coord(c.y, g.lo[1], g.inv_h, g.dim[1]),
-                         cell_coord(c.z, g.lo[2], g.inv_h, g.dim[2])};
-      int n = 0;
-      bool fits = true, covered = false;
-      for (int ob = 0; ob < a.n_offsets && fits; ob += 64) {
-        const float lbd = fmaxf(0.f, sqrtf((float)a.offsets[ob].w) * g.h - 2.f * g.slack);
-        if (lbd * lbd * (1.f - 1e-5f) > R2) {
-          covered = true;  // offsets are sorted by lb: the rest are beyond R too
-          break;
-        }
-        int cnt = 0, st = 0;
-        const int oi = ob + lane;
-        if (oi < a.n_offsets) {
-          const int4 o = a.offsets[oi];
-          const int ox = cc[0] + o.x, oy = cc[1] + o.y, oz = cc[2] + o.z;
-          if (ox >= 0 && ox < g.dim[0] && oy >= 0 && oy < g.dim[1] && oz >= 0 && oz < g.dim[2] &&
-              !(cell_bound(g, c, ox, oy, oz) > R2)) {
-            const int cell = cbase + ox + g.dim[0] * (oy + g.dim[1] * oz);
-            st = a.start[cell];
-            cnt = a.start[cell + 1] - st;
-          }
-        }
-        const int incl = wave_incl_sum(cnt);
-        const int excl = incl - cnt;
-        const int total = __shfl(incl, 63);
-        int carry = 0;
-        for (int f0 = 0; f0 < total && fits; f0 += 64) {
-          const int f = f0 + lane;
-          const int own = batch_owner(mark, lane, cnt, excl, f0, carry);
-          const int o_st = __shfl(st, own), o_ex = __shfl(excl, own);
-          bool keep = false;
-          float4 p = {0.f, 0.f, 0.f, 0.f};
-          if (f < total) {
-            const int idx = o_st + (f - o_ex);
-            p = a.sorted_pts[idx];
-            p.w = __uint_as_float((uint32_t)a.sorted_ids[idx]);
-            keep = !(dist2(p, c) > R2);
-          }
-          const uint64_t m = __ballot(keep);
-          const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-          const int add = __popcll(m);
-          if (n + add > kUnionCap) {
-            fits = false;
-          } else {
-            if (keep) uni[n + below] = p;
-            n += add;
-          }
-        }
-      }
-      // beyond the offset table the walk scans the whole instance: no list
-      if (fits && covered) {
-        wave_sync();
+      const int n = gather(c, R2, g, cbase);
+      if (n >= 0) {
         n_uni = n;
         uni_inst = inst;
         r0 = prev_r;
