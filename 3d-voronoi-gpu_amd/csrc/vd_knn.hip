@@ -453,7 +453,7 @@ constexpr int kQueryRun = VD_KNN_RUN;
 // grid, with the same keys and the same bounds, so their results are the
 // same; a run whose list would not fit walks as before.
 #ifndef VD_KNN_UNION_CAP
-#define VD_KNN_UNION_CAP 384
+#define VD_KNN_UNION_CAP 512
 #endif
 constexpr int kUnionCap = VD_KNN_UNION_CAP;
 
