@@ -935,6 +935,82 @@ __device__ __forceinline__ bool run_pair(const ClipArgs& a, Cell<PS, TS>& c, con
     // run their cuts together (the same operations in the same order per lane)
     int v = 0;
     bool go = K > 0;
+    // Without the arithmetic filter the trip's bisectors are tested as one
+    // batch (Cell::test_batch) and their outcomes taken in list order, as
+    // the tests one at a time below take them; only the cutting bisector is
+    // stored, at the next free plane position.
+    while (go && !c.filter) {
+      P4 e[kBatchTests];
+      float d2z[kBatchTests];
+      bool ok[kBatchTests];
+      uint32_t cf[kBatchTests];
+      const int n = min(kBatchTests, K - v);
+      const P4 s = c.seed;
+#pragma unroll
+      for (int k = 0; k < kBatchTests; ++k) {
+        const int z = nbr[min(v + k, K - 1)];
+        ok[k] = (unsigned)z < (unsigned)a.n_sites;
+        const P4 zp = site_at(a, ok[k] ? z : 0);
+        e[k] = bisector(s, zp);
+        const P4 d = sub3(zp, s);
+        d2z[k] = ok[k] ? dot3(d, d) : __int_as_float(0x7fc00000);
+        cf[k] = 0u;
+      }
+      if (c.nv >= c.pcap) {
+        // the vertex cap: the first test reports it (after its neighbour check)
+        if (!ok[0]) {  // defensive: never read outside the site array
+          c.status = kInconsistentBoundary;
+          return false;
+        }
+        c.status = kVertexOverflow;
+        if (!c.security_reached(d2z[0])) return false;
+        stopped = true;
+        break;
+      }
+      c.test_batch(e, d2z, n, cf);
+      int cut_k = -1;
+      bool bad = false;
+      P4 eq = e[0];
+      float dz = d2z[0];
+      uint32_t conf = 0u;
+#pragma unroll
+      for (int k = 0; k < kBatchTests; ++k) {
+        if (!go || cut_k >= 0) continue;
+        if (!ok[k]) {
+          bad = true;
+          go = false;
+        } else if (cf[k] != 0u) {
+          cut_k = k;
+          eq = e[k];
+          dz = d2z[k];
+          conf = cf[k];
+        } else if (d2z[k] > 4.f * c.vmax) {
+          stopped = true;
+          go = false;
+        } else if (++v == K) {
+          go = false;
+        }
+      }
+      if (bad) {  // defensive: never read outside the site array
+        c.status = kInconsistentBoundary;
+        return false;
+      }
+      if (cut_k >= 0) {
+        const int pid = c.nv;  // below pcap: not capped
+        c.ps.put(pid, eq);
+        ++c.nv;
+        c.nr = 0;
+        const int r = c.clip_cut(pid, conf, dz);
+        if (r >= 0 ? r != 0 : c.security_reached(dz)) {
+          stopped = true;
+          go = false;
+        } else if (c.status != kSuccess) {
+          return false;
+        } else if (++v == K) {
+          go = false;
+        }
+      }
+    }
     while (go) {
       bool cut = false;
       uint32_t conf = 0;
@@ -1658,7 +1734,10 @@ __global__ __launch_bounds__(64) void k_cull_site(ClipArgs a, int depth, unsigne
 // no_intersection; then a vertex-cap overflow goes to the retry list and
 // security_radius_not_reached to the deeper-list re-clip when they are on,
 // anything else is accumulated.
-constexpr int kMomentsBlock = 256;
+#ifndef VD_MOMENTS_BLOCK
+#define VD_MOMENTS_BLOCK 128
+#endif
+constexpr int kMomentsBlock = VD_MOMENTS_BLOCK;
 
 // Sum over the lanes of mask m (wave-uniform) of a per-lane value, in every
 // lane; 64-bit integers add exactly, so the order does not matter.
@@ -2366,10 +2445,13 @@ hipError_t clip_launch(const ClipArgs& a, int form, int moments, int acc_mode, i
   return site_volume_pass(a, moments, acc_mode, st);
 }
 
+#ifndef VD_MOMENTS_GRID
+#define VD_MOMENTS_GRID 8192
+#endif
 hipError_t site_volume_pass(const ClipArgs& a, int moments, int acc_mode, hipStream_t st) {
   if (a.n_pairs <= 0 || !a.rec_meta || a.rec_cap == 0) return hipSuccess;
   // the record count is read on the device
-  const dim3 grid((unsigned)std::min<long long>(((long long)a.rec_cap + kMomentsBlock - 1) / kMomentsBlock, 4096)),
+  const dim3 grid((unsigned)std::min<long long>(((long long)a.rec_cap + kMomentsBlock - 1) / kMomentsBlock, VD_MOMENTS_GRID)),
       block(kMomentsBlock);
   if (moments == 0 && acc_mode == 0)
     hipLaunchKernelGGL((k_site_moments<0, 0>), grid, block, 0, st, a);
