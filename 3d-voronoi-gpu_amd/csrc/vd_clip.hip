@@ -92,29 +92,21 @@ struct PlaneStoreShared {
 };
 
 // Kernel C: the stored cell's planes (ids 0..nvc-1) in a table shared by the
-// wave, the tet's face planes appended after them (ids nvc..nvc+3) in
-// registers.
+// wave, the tet's face planes appended after them (ids nvc..nvc+3) in a
+// per-lane LDS table; the id picks the address, so a fetch is one LDS read
+// (in registers, the selects among the four faces became an indexed array in
+// scratch memory).
 struct PlaneStoreCell {
   const float4* cpl;  // [nvc] shared
   int nvc;
-  P4 f0, f1, f2, f3;
+  float4* fl;         // [4][64] the lane's tet faces
+  int lane;
   __device__ __forceinline__ P4 get(int i) const {
-    const float4 b = cpl[i < nvc ? i : 0];
-    const int j = i - nvc;
-    float x = b.x, y = b.y, z = b.z, w = b.w;
-    x = j == 3 ? f3.x : x; y = j == 3 ? f3.y : y; z = j == 3 ? f3.z : z; w = j == 3 ? f3.w : w;
-    x = j == 2 ? f2.x : x; y = j == 2 ? f2.y : y; z = j == 2 ? f2.z : z; w = j == 2 ? f2.w : w;
-    x = j == 1 ? f1.x : x; y = j == 1 ? f1.y : y; z = j == 1 ? f1.z : z; w = j == 1 ? f1.w : w;
-    x = j == 0 ? f0.x : x; y = j == 0 ? f0.y : y; z = j == 0 ? f0.z : z; w = j == 0 ? f0.w : w;
-    return {x, y, z, w};
+    const float4* p = i < nvc ? cpl + i : fl + ((i - nvc) * 64 + lane);
+    const float4 v = *p;
+    return {v.x, v.y, v.z, v.w};
   }
-  __device__ __forceinline__ void put(int i, const P4& p) {
-    const int j = i - nvc;
-    if (j == 0) f0 = p;
-    if (j == 1) f1 = p;
-    if (j == 2) f2 = p;
-    if (j == 3) f3 = p;
-  }
+  __device__ __forceinline__ void put(int i, const P4& p) { fl[(i - nvc) * 64 + lane] = make_float4(p.x, p.y, p.z, p.w); }
 };
 
 // Volume pass of the site-major form: face planes of the lane (ids 0..3) in
@@ -2007,8 +1999,11 @@ __global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs) {
   uint32_t* tr = ctr + a.tcap;                              // [tcap][64]
   uint8_t* nx = (uint8_t*)(tr + (size_t)a.tcap * 64);       // [pcap][64]
   int* surv = (int*)(nx + (size_t)a.pcap * 64);             // [128] (offset is a multiple of 64)
+  float4* fl = (float4*)(((uintptr_t)(surv + 128) + 15) & ~(uintptr_t)15);  // [4][64]
   Cell<PlaneStoreCell, TriPlain> c;
   c.ps.cpl = cpl;
+  c.ps.fl = fl;
+  c.ps.lane = lane;
   c.ts.tr = tr;
   c.ts.lane = lane;
   c.nx = nx;
@@ -2494,7 +2489,7 @@ hipError_t srnr_rows(const unsigned* list, const unsigned* n_list, long long cap
 
 size_t cells_lds(int pcap, int tcap) { return (size_t)64 * (16 * pcap + 4 * tcap + pcap); }
 size_t cells_clip_lds(int pcap, int tcap) {
-  return (size_t)16 * pcap + 4 * (size_t)tcap + (size_t)64 * (4 * tcap + pcap) + 4 * 128;
+  return (size_t)16 * pcap + 4 * (size_t)tcap + (size_t)64 * (4 * tcap + pcap) + 4 * 128 + 16 + 16 * 4 * 64;
 }
 
 hipError_t cells_launch(const ClipArgs& a, const CellStore& cs, int moments, int acc_mode, hipStream_t st) {

@@ -253,7 +253,8 @@ int check_config(vd_ctx* ctx, const vd_config& c) {
     return ctx->fail(VD_ERR_ARG, "bad moments %d", c.moments);
   if (c.form != VD_FORM_TET_MAJOR && c.form != VD_FORM_SITE_MAJOR && c.form != VD_FORM_CELLS)
     return ctx->fail(VD_ERR_ARG, "bad form %d", c.form);
-  if (c.form == VD_FORM_CELLS && (vd::cells_lds(c.pcap, c.tcap) > 160 * 1024 || c.pcap > 250))
+  if (c.form == VD_FORM_CELLS &&
+      (vd::cells_lds(c.pcap, c.tcap) > 160 * 1024 || vd::cells_clip_lds(c.pcap, c.tcap) > 160 * 1024 || c.pcap > 250))
     return ctx->fail(VD_ERR_ARG, "pcap/tcap too large for the cell kernels' LDS");
   if (!(c.grid_density > 0.05f && c.grid_density < 1000.f))
     return ctx->fail(VD_ERR_ARG, "grid_density out of range");
