@@ -1622,6 +1622,11 @@ __global__ __launch_bounds__(64) void k_cull_site(ClipArgs a, int depth, unsigne
       if (lane == 0) pair_count[s] = 0;
       continue;
     }
+    if (a.cell_status && a.cell_status[s] != kSuccess && a.cell_status[s] != kSecurityRadiusNotReached) {
+      // no cell: every pair of the site takes the cell's status
+      if (lane == 0) pair_count[s] = pe - pb;
+      continue;
+    }
     const P4 sp = site_at(a, s);
     const int32_t* nbr = a.site_knn + (size_t)s * a.K;
     __syncthreads();  // the previous site's readers are done
@@ -2012,7 +2017,7 @@ __global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs) {
   c.tcap = a.tcap;
   c.filter = a.filter;
   for (int s = blockIdx.x; s < a.n_sites; s += gridDim.x) {
-    const unsigned pb = a.pair_start[s], pe = a.pair_start[s + 1];
+    const unsigned pb = a.pair_start[s], pe = a.pair_count ? pb + a.pair_count[s] : a.pair_start[s + 1];
     if (pb == pe) continue;
     const int cst = cs.status[s];
     const int cnv = cs.nv[s], cnt = cs.nt[s];
@@ -2492,14 +2497,26 @@ size_t cells_clip_lds(int pcap, int tcap) {
   return (size_t)16 * pcap + 4 * (size_t)tcap + (size_t)64 * (4 * tcap + pcap) + 4 * 128 + 16 + 16 * 4 * 64;
 }
 
-hipError_t cells_launch(const ClipArgs& a, const CellStore& cs, int moments, int acc_mode, hipStream_t st) {
-  if (a.n_sites <= 0) return hipSuccess;
-  size_t lds = cells_lds(a.pcap, a.tcap);
+hipError_t cells_launch(const ClipArgs& a0, const CellStore& cs, int moments, int acc_mode, hipStream_t st,
+                        unsigned* pair_ids, unsigned* cull_count) {
+  if (a0.n_sites <= 0) return hipSuccess;
+  size_t lds = cells_lds(a0.pcap, a0.tcap);
   hipError_t e = hipFuncSetAttribute((const void*)k_cells<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_cells<0>, dim3((unsigned)((a.n_sites + 63) / 64)), dim3(64), lds, st, a, cs);
+  hipLaunchKernelGGL(k_cells<0>, dim3((unsigned)((a0.n_sites + 63) / 64)), dim3(64), lds, st, a0, cs);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if (a.n_pairs <= 0) return hipSuccess;
+  if (a0.n_pairs <= 0) return hipSuccess;
+  ClipArgs a = a0;
+  if (a.cull_depth > 0 && pair_ids && cull_count) {
+    // A tet beyond one of the site's first bisectors (by the emptiness test's
+    // margin) misses the cell: kernel B clipped the cell by that bisector, or
+    // stopped earlier with every vertex on the seed's side of it, so clipping
+    // the cell by the tet leaves nothing either.  Kernel C takes the survivors.
+    ClipArgs ac = a;
+    ac.cell_status = cs.status;
+    if ((e = cull_sites(ac, pair_ids, cull_count, st)) != hipSuccess) return e;
+    a.pair_count = cull_count;
+  }
   lds = cells_clip_lds(a.pcap, a.tcap);
   auto go = [&](auto k) -> hipError_t {
     hipError_t e2 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -22,8 +22,10 @@ struct ClipArgs {
   int n_sites;
   const unsigned* pair_start;  // site-major form: n_sites + 1
   const unsigned* pair_ids;  // site-major form: n_pairs pair indices grouped by site
-  const unsigned* pair_count;  // site-major form, optional: only the first pair_count[s]
+  const unsigned* pair_count;  // site-major and cell forms, optional: only the first pair_count[s]
                                // pairs of site s's range are clipped (cull_sites' survivors)
+  const int* cell_status;      // cell form, optional: cull_sites keeps every pair of a site
+                               // whose cell was not built (its pairs take the cell's status)
   int pcap, tcap;            // polytope caps (planes, vertices) of this launch
   int tcap_full;             // configured vertex cap (site-major form: pairs that
                              // overflow a smaller launch cap go to retry_list)
@@ -82,8 +84,11 @@ constexpr int kFormTetCached = 3;
 size_t cells_lds(int pcap, int tcap);
 size_t cells_clip_lds(int pcap, int tcap);
 // Kernel B over all sites, then kernel C over the pairs grouped by site
-// (a.pair_start / a.pair_ids); a.cull_depth > 0 enables the ball test.
-hipError_t cells_launch(const ClipArgs& a, const CellStore& cs, int moments, int acc_mode, hipStream_t st);
+// (a.pair_start / a.pair_ids); a.cull_depth > 0 enables the ball test and,
+// with `cull_count` (n_sites), the emptiness pass (cull_sites over pair_ids)
+// between the two kernels.
+hipError_t cells_launch(const ClipArgs& a, const CellStore& cs, int moments, int acc_mode, hipStream_t st,
+                        unsigned* pair_ids = nullptr, unsigned* cull_count = nullptr);
 // LDS of the site-major kernel with `depth` bisectors per site table.
 size_t clip_site_lds(int depth, int tcap);
 // Triangle words per record of the site-major form's volume pass.

@@ -794,7 +794,9 @@ int vd_clip_step(vd_ctx* ctx) {
       VD_HIP(ctx->c_tris.ensure((size_t)n * c.tcap), "alloc cells");
       vd::CellStore cs{ctx->c_status.p, ctx->c_nv.p, ctx->c_nt.p, ctx->c_r2.p, ctx->c_planes.p, ctx->c_tris.p};
       a.cull_depth = c.cull_depth;
-      VD_HIP(vd::cells_launch(a, cs, c.moments, c.accum, ctx->stream), "cells");
+      if (c.cull_depth > 0) VD_HIP(ctx->p_surv.ensure(n), "alloc pairs");
+      VD_HIP(vd::cells_launch(a, cs, c.moments, c.accum, ctx->stream, ctx->p_ids.p, c.cull_depth > 0 ? ctx->p_surv.p : nullptr),
+             "cells");
       VD_HIP(record_phase(ctx, (5)), "event");
       VD_HIP(record_phase(ctx, (3)), "event");
       ctx->ev_clip = true;
