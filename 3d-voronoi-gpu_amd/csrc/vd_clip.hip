@@ -158,6 +158,7 @@ struct PlaneStoreTable {
 //   cached value is the one the recomputation would give, bit for bit.
 struct TriPlain {
   static constexpr bool kCached = false;
+  static constexpr bool kHasD2 = false;
   uint32_t* tr;  // [tcap][64]
   int lane;
   __device__ __forceinline__ uint32_t get(int t) const { return tr[t * 64 + lane]; }
@@ -172,6 +173,7 @@ struct TriPlain {
 };
 struct TriCached {
   static constexpr bool kCached = true;
+  static constexpr bool kHasD2 = true;
   uint2* tr;  // [tcap][64]: (plane triple, d2 bits)
   int lane;
   __device__ __forceinline__ uint32_t get(int t) const { return tr[t * 64 + lane].x; }
@@ -192,9 +194,19 @@ struct TriCached {
   __device__ __forceinline__ void copy(int dst, int src) { tr[dst * 64 + lane] = tr[src * 64 + lane]; }
 };
 
+// TriPlainD2: the layout of TriCached (plane triple + the vertex's squared
+// distance to the seed) for the plain clip, whose vertex count is not bound
+// by the cached clip's 32-bit vertex masks: the distances serve the
+// security-radius test only.
+struct TriPlainD2 : TriCached {
+  static constexpr bool kCached = false;
+  static constexpr bool kHasD2 = true;
+};
+
 // TriRecord: the triangles of a volume-pass record (read only).
 struct TriRecord {
   static constexpr bool kCached = false;
+  static constexpr bool kHasD2 = false;
   const uint32_t* tr;
   __device__ __forceinline__ uint32_t get(int t) const { return tr[t]; }
 };
@@ -269,7 +281,7 @@ struct Cell {
   // Stores triangle t; the cached form also computes its vertex distance,
   // which it returns.
   __device__ __forceinline__ float put_tri(int t, uint32_t v) {
-    if constexpr (TS::kCached) {
+    if constexpr (TS::kHasD2) {
       const float d2 = seed_d2(vertex(v));
       ts.put(t, v, d2);
       return d2;
@@ -348,18 +360,27 @@ struct Cell {
     ps.put(4, {0.f, 0.f, 1.f, -lo});
     ps.put(5, {0.f, 0.f, -1.f, hi});
     nv = 6;
-    put_tri(0, pack3(2, 5, 0));
-    put_tri(1, pack3(5, 3, 0));
-    put_tri(2, pack3(1, 5, 2));
-    put_tri(3, pack3(5, 1, 3));
-    put_tri(4, pack3(4, 2, 0));
-    put_tri(5, pack3(4, 0, 3));
-    put_tri(6, pack3(2, 4, 1));
-    put_tri(7, pack3(4, 3, 1));
+    float m = put_tri(0, pack3(2, 5, 0));
+    m = fmaxf(put_tri(1, pack3(5, 3, 0)), m);
+    m = fmaxf(put_tri(2, pack3(1, 5, 2)), m);
+    m = fmaxf(put_tri(3, pack3(5, 1, 3)), m);
+    m = fmaxf(put_tri(4, pack3(4, 2, 0)), m);
+    m = fmaxf(put_tri(5, pack3(4, 0, 3)), m);
+    m = fmaxf(put_tri(6, pack3(2, 4, 1)), m);
+    m = fmaxf(put_tri(7, pack3(4, 3, 1)), m);
     nt = 8;
     first = kNoPlane;
     far = 0xFFu;
     last_d2z = 0.f;
+    // the cached form's largest vertex distance and, for its bisector box
+    // test, a box holding every vertex (relative to the seed): the domain
+    vmax = m;
+    blo[0] = lo - s.x;
+    blo[1] = lo - s.y;
+    blo[2] = lo - s.z;
+    bhi[0] = hi - s.x;
+    bhi[1] = hi - s.y;
+    bhi[2] = hi - s.z;
   }
 
   // Every vertex closer to the seed than half the distance to z
@@ -367,7 +388,7 @@ struct Cell {
   __device__ bool security_reached(float d2z) const {
     float vmax = 0.f;
     for (int t = 0; t < nt; ++t) {
-      if constexpr (TS::kCached)
+      if constexpr (TS::kHasD2)
         vmax = fmaxf(ts.d2(t), vmax);
       else
         vmax = fmaxf(seed_d2(vertex(tri(t))), vmax);
@@ -1943,10 +1964,13 @@ template <int U>
 __global__ __launch_bounds__(64) void k_cells(ClipArgs a, CellStore cs) {
   extern __shared__ __align__(16) uint8_t smem[];
   const int lane = threadIdx.x;
-  Cell<PlaneStoreLocal, TriPlain> c;
+  // the plain clip with the vertices' distances stored: the security test
+  // after every bisector reads them instead of solving every vertex again
+  // (the cached clip's vertex masks hold 32 vertices; a cell has more)
+  Cell<PlaneStoreLocal, TriPlainD2> c;
   c.ps.pl = (float4*)smem;
   c.ps.lane = lane;
-  c.ts.tr = (uint32_t*)(c.ps.pl + (size_t)a.pcap * 64);
+  c.ts.tr = (uint2*)(c.ps.pl + (size_t)a.pcap * 64);
   c.ts.lane = lane;
   c.nx = (uint8_t*)(c.ts.tr + (size_t)a.tcap * 64);
   c.lane = lane;
@@ -1983,7 +2007,7 @@ __global__ __launch_bounds__(64) void k_cells(ClipArgs a, CellStore cs) {
     if (!stopped && c.status == kSuccess) c.status = kSecurityRadiusNotReached;
     // store: header, planes, triangles, farthest vertex
     float r2 = 0.f;
-    for (int t = 0; t < c.nt; ++t) r2 = fmaxf(r2, c.seed_d2(c.vertex(c.tri(t))));
+    for (int t = 0; t < c.nt; ++t) r2 = fmaxf(r2, c.ts.d2(t));
     cs.status[s] = c.status;
     cs.nv[s] = min(c.nv, a.pcap);
     cs.nt[s] = c.nt;
@@ -2492,7 +2516,7 @@ hipError_t srnr_rows(const unsigned* list, const unsigned* n_list, long long cap
   return hipGetLastError();
 }
 
-size_t cells_lds(int pcap, int tcap) { return (size_t)64 * (16 * pcap + 4 * tcap + pcap); }
+size_t cells_lds(int pcap, int tcap) { return (size_t)64 * (16 * pcap + 8 * tcap + pcap); }
 size_t cells_clip_lds(int pcap, int tcap) {
   return (size_t)16 * pcap + 4 * (size_t)tcap + (size_t)64 * (4 * tcap + pcap) + 4 * 128 + 16 + 16 * 4 * 64;
 }
