@@ -2023,14 +2023,15 @@ template <int MF, int ACC>
 __global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs) {
   extern __shared__ __align__(16) uint8_t smem[];
   const int lane = threadIdx.x;
-  float4* cpl = (float4*)smem;                              // [pcap]
-  uint32_t* ctr = (uint32_t*)(cpl + a.pcap);                // [tcap]
-  uint32_t* tr = ctr + a.tcap;                              // [tcap][64]
+  // two cell slots, so that the pairs of consecutive sites share batches
+  float4* cpl = (float4*)smem;                              // [2][pcap]
+  uint32_t* ctr = (uint32_t*)(cpl + 2 * a.pcap);            // [2][tcap]
+  uint32_t* tr = ctr + 2 * a.tcap;                          // [tcap][64]
   uint8_t* nx = (uint8_t*)(tr + (size_t)a.tcap * 64);       // [pcap][64]
-  int* surv = (int*)(nx + (size_t)a.pcap * 64);             // [128] (offset is a multiple of 64)
-  float4* fl = (float4*)(((uintptr_t)(surv + 128) + 15) & ~(uintptr_t)15);  // [4][64]
+  int* surv = (int*)(nx + (size_t)a.pcap * 64);             // [128] queued pairs
+  uint8_t* sslot = (uint8_t*)(surv + 128);                  // [128] their slots
+  float4* fl = (float4*)(((uintptr_t)(sslot + 128) + 15) & ~(uintptr_t)15);  // [4][64]
   Cell<PlaneStoreCell, TriPlain> c;
-  c.ps.cpl = cpl;
   c.ps.fl = fl;
   c.ps.lane = lane;
   c.ts.tr = tr;
@@ -2040,6 +2041,86 @@ __global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs) {
   c.pcap = a.pcap;
   c.tcap = a.tcap;
   c.filter = a.filter;
+  // wave-uniform: the site of each slot, its cell status and counts, and the
+  // queued pairs of each slot
+  // (scalars, not arrays: a slot is picked at run time, and an indexed array
+  // would go to scratch memory)
+  int site0 = 0, site1 = 0, cst0 = kSuccess, cst1 = kSuccess, cnv0 = 0, cnv1 = 0, cnt0 = 0, cnt1 = 0;
+  int queued = 0, in_q0 = 0, in_q1 = 0;
+  int slot = 1;  // slot of the current site
+
+  // Clips the queued pairs [0, n) (n <= 64), one per lane, each with its
+  // slot's cell; the rest of the queue moves to the front.
+  auto clip_batch = [&](int n) {
+    const bool live = lane < n;
+    const unsigned pair = live ? (unsigned)surv[lane] : 0u;
+    const int sl = live ? (int)sslot[lane] : 0;
+    const int rest = lane + 64 < queued ? surv[lane + 64] : 0;
+    const uint8_t rest_sl = lane + 64 < queued ? sslot[lane + 64] : (uint8_t)0;
+    __syncthreads();
+    if (lane + 64 < queued) {
+      surv[lane] = rest;
+      sslot[lane] = rest_sl;
+    }
+    __syncthreads();
+    in_q0 -= __popcll(__ballot(live && sl == 0));
+    in_q1 -= __popcll(__ballot(live && sl == 1));
+    queued -= n;
+    const int s = sl ? site1 : site0;
+    const int cst = sl ? cst1 : cst0;
+    const int cnv = sl ? cnv1 : cnv0;
+    const int cnt = sl ? cnt1 : cnt0;
+    const uint32_t* ct = ctr + sl * a.tcap;
+    int status = kNoIntersection;
+    if (live) {
+      P4 v[4];
+      tet_corners(a, (int)(pair / (unsigned)a.TK), v);
+      c.ps.cpl = cpl + sl * a.pcap;
+      c.ps.nvc = cnv;
+      c.seed = site_at(a, s);
+      c.status = cst;
+      c.keep_srnr = true;
+      c.nv = cnv;
+      c.nt = cnt;
+      c.first = kNoPlane;
+      for (int t = 0; t < cnt; ++t) c.ts.put(t, ct[t], 0.f);
+      static constexpr int face[4][3] = {{2, 1, 3}, {0, 2, 3}, {1, 0, 3}, {0, 1, 2}};
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        const P4 eq = tri2plane(v[face[f][0]], v[face[f][1]], v[face[f][2]]);
+        const int pid = c.nv;
+        if (pid < a.pcap) c.ps.put(pid, eq);
+        (void)c.clip(pid, eq, 0.f);
+        if (c.status == kNoIntersection) break;
+      }
+      PairSums<ACC> S;
+      bool acc = false;
+      if (c.status != kNoIntersection) {
+        if (MF == 0)
+          c.template moments_ref<ACC>(S);
+        else
+          c.template moments_robust<ACC>(S, v, cnv);  // planes below cnv: the cell's
+        if (!(isfinite((double)S.v) && isfinite((double)S.x) && isfinite((double)S.y) && isfinite((double)S.z)))
+          c.status = kNeedsExactPredicates;
+        else if (fabs((double)S.v) < 0.1)
+          c.status = kNoIntersection;
+        else
+          acc = true;
+      }
+      status = c.status;
+      if (acc) accumulate<ACC>(a, s, S);
+      if (a.pair_status) a.pair_status[pair] = (uint8_t)status;
+      if (a.pair_vals) {
+        double* pv = a.pair_vals + 4 * (size_t)pair;
+        pv[0] = acc ? (double)S.v : 0.0;
+        pv[1] = acc ? (double)S.x : 0.0;
+        pv[2] = acc ? (double)S.y : 0.0;
+        pv[3] = acc ? (double)S.z : 0.0;
+      }
+    }
+    count_status(a, status, live);
+  };
+
   for (int s = blockIdx.x; s < a.n_sites; s += gridDim.x) {
     const unsigned pb = a.pair_start[s], pe = a.pair_count ? pb + a.pair_count[s] : a.pair_start[s + 1];
     if (pb == pe) continue;
@@ -2061,66 +2142,28 @@ __global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs) {
       if (a.status_hist && lane == 0) atomicAdd(a.status_hist + cst, (unsigned long long)(pe - pb));
       continue;
     }
+    // the other slot takes this site's cell once no queued pair refers to it
+    slot ^= 1;
+    if ((slot ? in_q1 : in_q0) > 0) clip_batch(queued);  // (queued < 64 here)
     __syncthreads();
-    for (int i = lane; i < cnv; i += 64) cpl[i] = cs.planes[(size_t)s * a.pcap + i];
-    for (int t = lane; t < cnt; t += 64) ctr[t] = cs.tris[(size_t)s * a.tcap + t];
+    for (int i = lane; i < cnv; i += 64) cpl[slot * a.pcap + i] = cs.planes[(size_t)s * a.pcap + i];
+    for (int t = lane; t < cnt; t += 64) ctr[slot * a.tcap + t] = cs.tris[(size_t)s * a.tcap + t];
     __syncthreads();
-    c.ps.nvc = cnv;
+    if (slot) {
+      site1 = s;
+      cst1 = cst;
+      cnv1 = cnv;
+      cnt1 = cnt;
+    } else {
+      site0 = s;
+      cst0 = cst;
+      cnv0 = cnv;
+      cnt0 = cnt;
+    }
     // the cell lies in the ball of radius sqrt(cr2) around the site; a tet
     // whose bounding ball around its centroid misses that ball by more than
     // the margin has an empty intersection with the cell
     const float cr = sqrtf(cr2) * (1.f + 1e-5f) + kCullMargin;
-
-    auto clip_batch = [&](unsigned pair, bool live) {
-      int status = kNoIntersection;
-      if (live) {
-        P4 v[4];
-        tet_corners(a, (int)(pair / (unsigned)a.TK), v);
-        c.seed = sp;
-        c.status = cst;
-        c.keep_srnr = true;
-        c.nv = cnv;
-        c.nt = cnt;
-        c.first = kNoPlane;
-        for (int t = 0; t < cnt; ++t) c.ts.put(t, ctr[t], 0.f);
-        static constexpr int face[4][3] = {{2, 1, 3}, {0, 2, 3}, {1, 0, 3}, {0, 1, 2}};
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-          const P4 eq = tri2plane(v[face[f][0]], v[face[f][1]], v[face[f][2]]);
-          const int pid = c.nv;
-          if (pid < a.pcap) c.ps.put(pid, eq);
-          (void)c.clip(pid, eq, 0.f);
-          if (c.status == kNoIntersection) break;
-        }
-        PairSums<ACC> S;
-        bool acc = false;
-        if (c.status != kNoIntersection) {
-          if (MF == 0)
-            c.template moments_ref<ACC>(S);
-          else
-            c.template moments_robust<ACC>(S, v, cnv);  // planes below cnv: the cell's
-          if (!(isfinite((double)S.v) && isfinite((double)S.x) && isfinite((double)S.y) && isfinite((double)S.z)))
-            c.status = kNeedsExactPredicates;
-          else if (fabs((double)S.v) < 0.1)
-            c.status = kNoIntersection;
-          else
-            acc = true;
-        }
-        status = c.status;
-        if (acc) accumulate<ACC>(a, s, S);
-        if (a.pair_status) a.pair_status[pair] = (uint8_t)status;
-        if (a.pair_vals) {
-          double* pv = a.pair_vals + 4 * (size_t)pair;
-          pv[0] = acc ? (double)S.v : 0.0;
-          pv[1] = acc ? (double)S.x : 0.0;
-          pv[2] = acc ? (double)S.y : 0.0;
-          pv[3] = acc ? (double)S.z : 0.0;
-        }
-      }
-      count_status(a, status, live);
-    };
-
-    int queued = 0;
     for (unsigned base = pb; base < pe; base += 64) {
       const unsigned i = base + lane;
       const bool live = i < pe;
@@ -2155,24 +2198,18 @@ __global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs) {
       const bool keep = live && !culled;
       const unsigned long long mask = __ballot(keep);
       const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
-      if (keep) surv[queued + below] = (int)pair;
-      queued += __popcll(mask);
-      __syncthreads();
-      if (queued >= 64) {
-        const unsigned mine = (unsigned)surv[lane];
-        const int rest = lane + 64 < queued ? surv[lane + 64] : 0;
-        __syncthreads();
-        if (lane + 64 < queued) surv[lane] = rest;
-        queued -= 64;
-        __syncthreads();
-        clip_batch(mine, true);
+      if (keep) {
+        surv[queued + below] = (int)pair;
+        sslot[queued + below] = (uint8_t)slot;
       }
-    }
-    if (queued > 0) {
-      const unsigned mine = lane < queued ? (unsigned)surv[lane] : 0u;
-      clip_batch(mine, lane < queued);
+      queued += __popcll(mask);
+      if (slot) in_q1 += __popcll(mask);
+      else in_q0 += __popcll(mask);
+      __syncthreads();
+      if (queued >= 64) clip_batch(64);
     }
   }
+  if (queued > 0) clip_batch(queued);
 }
 
 // Pairs grouped by seed: a counting sort over sites.  A block takes a run of
@@ -2518,8 +2555,11 @@ hipError_t srnr_rows(const unsigned* list, const unsigned* n_list, long long cap
 
 size_t cells_lds(int pcap, int tcap) { return (size_t)64 * (16 * pcap + 8 * tcap + pcap); }
 size_t cells_clip_lds(int pcap, int tcap) {
-  return (size_t)16 * pcap + 4 * (size_t)tcap + (size_t)64 * (4 * tcap + pcap) + 4 * 128 + 16 + 16 * 4 * 64;
+  return (size_t)32 * pcap + 8 * (size_t)tcap + (size_t)64 * (4 * tcap + pcap) + 5 * 128 + 16 + 16 * 4 * 64;
 }
+
+// Workgroups (one wave each) of kernel C.
+constexpr int kCellsClipBlocks = 4096;
 
 hipError_t cells_launch(const ClipArgs& a0, const CellStore& cs, int moments, int acc_mode, hipStream_t st,
                         unsigned* pair_ids, unsigned* cull_count) {
@@ -2545,7 +2585,8 @@ hipError_t cells_launch(const ClipArgs& a0, const CellStore& cs, int moments, in
   auto go = [&](auto k) -> hipError_t {
     hipError_t e2 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e2 != hipSuccess) return e2;
-    hipLaunchKernelGGL(k, dim3((unsigned)a.n_sites), dim3(64), lds, st, a, cs);
+    // each wave takes every kCellsClipBlocks-th site (a few dozen at C2's size)
+    hipLaunchKernelGGL(k, dim3((unsigned)std::min(a.n_sites, kCellsClipBlocks)), dim3(64), lds, st, a, cs);
     return hipGetLastError();
   };
   if (moments == 0 && acc_mode == 0) return go(k_cells_clip<0, 0>);
