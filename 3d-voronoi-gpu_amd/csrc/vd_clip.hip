@@ -2558,8 +2558,14 @@ size_t cells_clip_lds(int pcap, int tcap) {
   return (size_t)32 * pcap + 8 * (size_t)tcap + (size_t)64 * (4 * tcap + pcap) + 5 * 128 + 16 + 16 * 4 * 64;
 }
 
-// Workgroups (one wave each) of kernel C.
-constexpr int kCellsClipBlocks = 4096;
+// Workgroups (one wave each) of kernel C: 8,192 at 1M tets / 100k sites
+// (mean clipping ms over iterations 1-3: 1,024 → 28.3, 4,096 → 28.1, 8,192
+// → 27.4-27.6, 16,384 → 27.5; contiguous stretches of the grid order per
+// wave instead of every 8,192nd site: 27.4-27.6).
+#ifndef VD_CELLS_CLIP_BLOCKS
+#define VD_CELLS_CLIP_BLOCKS 8192
+#endif
+constexpr int kCellsClipBlocks = VD_CELLS_CLIP_BLOCKS;
 
 hipError_t cells_launch(const ClipArgs& a0, const CellStore& cs, int moments, int acc_mode, hipStream_t st,
                         unsigned* pair_ids, unsigned* cull_count) {
@@ -2585,7 +2591,7 @@ hipError_t cells_launch(const ClipArgs& a0, const CellStore& cs, int moments, in
   auto go = [&](auto k) -> hipError_t {
     hipError_t e2 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e2 != hipSuccess) return e2;
-    // each wave takes every kCellsClipBlocks-th site (a few dozen at C2's size)
+    // each wave takes every kCellsClipBlocks-th site
     hipLaunchKernelGGL(k, dim3((unsigned)std::min(a.n_sites, kCellsClipBlocks)), dim3(64), lds, st, a, cs);
     return hipGetLastError();
   };
