@@ -1957,8 +1957,9 @@ __global__ __launch_bounds__(kMomentsBlock) void k_site_moments(ClipArgs a) {
 // stored for kernel C with the squared radius of its farthest vertex.
 // Kernel C (voronoi.cu:620-645, 703-726): for every (tet, site) pair, the
 // stored cell clipped by the tet's four face planes, then volume and moments
-// as in kernel A.  Site-major: a wavefront takes a site, loads its cell into
-// LDS and takes its pairs 64 at a time.
+// as in kernel A.  Site-major: a wavefront takes its sites one after another,
+// loads each site's cell into one of two LDS slots and queues the site's
+// pairs; every 64 queued pairs (of one or two sites) are clipped together.
 
 template <int U>
 __global__ __launch_bounds__(64) void k_cells(ClipArgs a, CellStore cs) {
