@@ -1643,8 +1643,14 @@ __global__ __launch_bounds__(64) void k_cull_site(ClipArgs a, int depth, unsigne
       if (lane == 0) pair_count[s] = 0;
       continue;
     }
-    if (a.cell_status && a.cell_status[s] != kSuccess && a.cell_status[s] != kSecurityRadiusNotReached) {
-      // no cell: every pair of the site takes the cell's status
+    // Cell form: no cell (every pair of the site takes the cell's status), or
+    // a cell whose clip by a skipped pair's tet could overflow the caps: a
+    // culled pair's cell is cut by at most three of the tet's faces before
+    // one empties it, which adds at most 3 planes and 6 triangles (twice the
+    // planes less 4), and where that could reach a cap the reference reports
+    // the overflow.  Every pair is kept.
+    if (a.cell_status && ((a.cell_status[s] != kSuccess && a.cell_status[s] != kSecurityRadiusNotReached) ||
+                          a.cell_nv[s] + 4 > a.pcap || a.cell_nt[s] + 6 >= a.tcap)) {
       if (lane == 0) pair_count[s] = pe - pb;
       continue;
     }
@@ -1961,8 +1967,23 @@ __global__ __launch_bounds__(kMomentsBlock) void k_site_moments(ClipArgs a) {
 // loads each site's cell into one of two LDS slots and queues the site's
 // pairs; every 64 queued pairs (of one or two sites) are clipped together.
 
+// Kernel B's launches: every site at caps of at most kCellsFastPcap planes
+// and kCellsFastTcap vertices (76 KB of LDS per wave, two waves per CU
+// instead of one at 64/96), then the sites whose cell failed there (the
+// redo list, about 4 in 100k seeded cells at 40/64) at the configured caps.
+// A cell that succeeds at the smaller caps ran the same steps the configured
+// caps run (a cap only ends a build), so it is the same cell; every other
+// outcome is built again, since a runaway boundary walk's guard depends on
+// the plane cap.
+constexpr int kCellsFastPcap = 40;
+constexpr int kCellsFastTcap = 64;
+
+// pstride / tstride: the cell store's planes and triangles per site (the
+// configured caps); redo_out: the sites whose cell failed are listed there
+// ([0] count, then ids); redo_in: build only the listed sites.
 template <int U>
-__global__ __launch_bounds__(64) void k_cells(ClipArgs a, CellStore cs) {
+__global__ __launch_bounds__(64) void k_cells(ClipArgs a, CellStore cs, int pstride, int tstride, unsigned* redo_out,
+                                              const unsigned* redo_in) {
   extern __shared__ __align__(16) uint8_t smem[];
   const int lane = threadIdx.x;
   // the plain clip with the vertices' distances stored: the security test
@@ -1979,9 +2000,10 @@ __global__ __launch_bounds__(64) void k_cells(ClipArgs a, CellStore cs) {
   c.tcap = a.tcap;
   c.filter = a.filter;
   const int K = a.K;
-  for (int base = blockIdx.x * 64; base < a.n_sites; base += gridDim.x * 64) {
-    const int s = base + lane;
-    if (s >= a.n_sites) continue;
+  const int n_items = redo_in ? (int)min(redo_in[0], (unsigned)a.n_sites) : a.n_sites;
+  for (int base = blockIdx.x * 64; base < n_items; base += gridDim.x * 64) {
+    if (base + lane >= n_items) continue;
+    const int s = redo_in ? (int)redo_in[1 + base + lane] : base + lane;
     const P4 sp = site_at(a, s);
     c.init_box(sp);
     const int32_t* nbr = a.site_knn + (size_t)s * K;
@@ -2013,10 +2035,12 @@ __global__ __launch_bounds__(64) void k_cells(ClipArgs a, CellStore cs) {
     cs.nv[s] = min(c.nv, a.pcap);
     cs.nt[s] = c.nt;
     cs.r2[s] = r2;
-    float4* pl = cs.planes + (size_t)s * a.pcap;
+    float4* pl = cs.planes + (size_t)s * pstride;
     for (int i = 0; i < min(c.nv, a.pcap); ++i) pl[i] = c.ps.pl[i * 64 + lane];
-    uint32_t* tr = cs.tris + (size_t)s * a.tcap;
+    uint32_t* tr = cs.tris + (size_t)s * tstride;
     for (int t = 0; t < c.nt; ++t) tr[t] = c.tri(t);
+    if (redo_out && c.status != kSuccess && c.status != kSecurityRadiusNotReached)
+      redo_out[1 + atomicAdd(redo_out, 1u)] = (unsigned)s;  // each site once: below n_sites entries
   }
 }
 
@@ -2163,14 +2187,16 @@ __global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs) {
     }
     // the cell lies in the ball of radius sqrt(cr2) around the site; a tet
     // whose bounding ball around its centroid misses that ball by more than
-    // the margin has an empty intersection with the cell
+    // the margin has an empty intersection with the cell (skipped only where
+    // the clip could not overflow the caps first, as in cull_sites)
     const float cr = sqrtf(cr2) * (1.f + 1e-5f) + kCullMargin;
+    const bool may_skip = cnv + 4 <= a.pcap && cnt + 6 < a.tcap;
     for (unsigned base = pb; base < pe; base += 64) {
       const unsigned i = base + lane;
       const bool live = i < pe;
       const unsigned pair = live ? a.pair_ids[i] : 0u;
       bool culled = false;
-      if (live && a.cull_depth > 0) {
+      if (live && a.cull_depth > 0 && may_skip) {
         P4 v[4];
         tet_corners(a, (int)(pair / (unsigned)a.TK), v);
         P4 g = {0.f, 0.f, 0.f, 0.f};
@@ -2569,13 +2595,27 @@ size_t cells_clip_lds(int pcap, int tcap) {
 constexpr int kCellsClipBlocks = VD_CELLS_CLIP_BLOCKS;
 
 hipError_t cells_launch(const ClipArgs& a0, const CellStore& cs, int moments, int acc_mode, hipStream_t st,
-                        unsigned* pair_ids, unsigned* cull_count) {
+                        unsigned* pair_ids, unsigned* cull_count, int fast_pcap, int fast_tcap) {
   if (a0.n_sites <= 0) return hipSuccess;
   size_t lds = cells_lds(a0.pcap, a0.tcap);
   hipError_t e = hipFuncSetAttribute((const void*)k_cells<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_cells<0>, dim3((unsigned)((a0.n_sites + 63) / 64)), dim3(64), lds, st, a0, cs);
+  ClipArgs ab = a0;
+  ab.pcap = std::min(a0.pcap, fast_pcap > 0 ? fast_pcap : kCellsFastPcap);
+  ab.tcap = std::min(a0.tcap, fast_tcap > 0 ? fast_tcap : kCellsFastTcap);
+  const bool redo = cs.redo && (ab.pcap < a0.pcap || ab.tcap < a0.tcap);
+  if (!redo) ab = a0;
+  if (redo && (e = fill_u32(cs.redo, 1, 0u, st)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cells<0>, dim3((unsigned)((a0.n_sites + 63) / 64)), dim3(64), cells_lds(ab.pcap, ab.tcap), st, ab,
+                     cs, a0.pcap, a0.tcap, redo ? cs.redo : nullptr, (const unsigned*)nullptr);
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (redo) {
+    // the listed sites at the configured caps (64 workgroups, looping over
+    // the list when it is longer)
+    hipLaunchKernelGGL(k_cells<0>, dim3(64), dim3(64), lds, st, a0, cs, a0.pcap, a0.tcap, (unsigned*)nullptr,
+                       (const unsigned*)cs.redo);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   if (a0.n_pairs <= 0) return hipSuccess;
   ClipArgs a = a0;
   if (a.cull_depth > 0 && pair_ids && cull_count) {
@@ -2585,6 +2625,8 @@ hipError_t cells_launch(const ClipArgs& a0, const CellStore& cs, int moments, in
     // the cell by the tet leaves nothing either.  Kernel C takes the survivors.
     ClipArgs ac = a;
     ac.cell_status = cs.status;
+    ac.cell_nv = cs.nv;
+    ac.cell_nt = cs.nt;
     if ((e = cull_sites(ac, pair_ids, cull_count, st)) != hipSuccess) return e;
     a.pair_count = cull_count;
   }

@@ -25,7 +25,9 @@ struct ClipArgs {
   const unsigned* pair_count;  // site-major and cell forms, optional: only the first pair_count[s]
                                // pairs of site s's range are clipped (cull_sites' survivors)
   const int* cell_status;      // cell form, optional: cull_sites keeps every pair of a site
-                               // whose cell was not built (its pairs take the cell's status)
+                               // whose cell was not built (its pairs take the cell's status),
+  const int* cell_nv;          // or whose cell's plane / triangle counts could overflow the
+  const int* cell_nt;          // caps while kernel C clips it (with cell_status)
   int pcap, tcap;            // polytope caps (planes, vertices) of this launch
   int tcap_full;             // configured vertex cap (site-major form: pairs that
                              // overflow a smaller launch cap go to retry_list)
@@ -74,6 +76,7 @@ struct CellStore {
   float* r2;
   float4* planes;     // n_sites x pcap
   uint32_t* tris;     // n_sites x tcap
+  unsigned* redo;     // optional, n_sites + 1: kernel B's list of cells built again at the configured caps
 };
 
 size_t clip_plain_lds(int pcap, int tcap);
@@ -87,8 +90,12 @@ size_t cells_clip_lds(int pcap, int tcap);
 // (a.pair_start / a.pair_ids); a.cull_depth > 0 enables the ball test and,
 // with `cull_count` (n_sites), the emptiness pass (cull_sites over pair_ids)
 // between the two kernels.
+// Kernel B builds every cell at caps fast_pcap / fast_tcap first (0: 40 and
+// 64, at most a.pcap / a.tcap) and, with cs.redo, the cells that failed
+// there again at a.pcap / a.tcap.
 hipError_t cells_launch(const ClipArgs& a, const CellStore& cs, int moments, int acc_mode, hipStream_t st,
-                        unsigned* pair_ids = nullptr, unsigned* cull_count = nullptr);
+                        unsigned* pair_ids = nullptr, unsigned* cull_count = nullptr, int fast_pcap = 0,
+                        int fast_tcap = 0);
 // LDS of the site-major kernel with `depth` bisectors per site table.
 size_t clip_site_lds(int depth, int tcap);
 // Triangle words per record of the site-major form's volume pass.
@@ -111,9 +118,19 @@ hipError_t site_volume_pass(const ClipArgs& a, int moments, int acc_mode, hipStr
 hipError_t pairs_by_site(const int32_t* tet_knn, long long n_pairs, int n_sites, int* count, unsigned* start,
                          int* cursor, unsigned* pair_ids, unsigned* scan_tmp, hipStream_t st);
 
+// Largest emptiness-test depth at which a skipped pair is what the reference
+// reports at caps (pcap, tcap): before the bisector that empties a tet's
+// polytope it has made at most depth - 1 cuts, so it has at most depth + 3
+// planes and 2 depth + 2 triangles (a closed polytope of this form has twice
+// its planes less 4 triangles), and neither cap can be reached.
+inline int exact_cull_depth(int depth, int pcap, int tcap) {
+  const int d = depth < pcap - 4 ? depth : pcap - 4;
+  return d < (tcap - 3) / 2 ? d : (tcap - 3) / 2;
+}
+
 // The site-major form's emptiness test as a pass of its own over the pairs
 // grouped by site (a.pair_start, pair_ids), with a.cull_depth bisectors (at
-// most 256): the survivors of site s are moved, in order, to the front of its
+// most 256; at most exact_cull_depth of the configured caps): the survivors of site s are moved, in order, to the front of its
 // range and their number written to pair_count[s]; a culled pair counts as
 // no_intersection (a.status_hist, a.pair_status, a.pair_vals).
 hipError_t cull_sites(const ClipArgs& a, unsigned* pair_ids, unsigned* pair_count, hipStream_t st);

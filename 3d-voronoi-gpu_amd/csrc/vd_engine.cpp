@@ -190,6 +190,7 @@ struct vd_ctx {
   DevBuf<float> c_r2;
   DevBuf<float4> c_planes;
   DevBuf<uint32_t> c_tris;
+  DevBuf<unsigned> c_redo;  // kernel B's list of cells built again at the configured caps
 
   // HIP events of the phases of the last kRing iterations: slot = iteration % kRing;
   // [0] start, [1] after kNN, [2] after pair grouping, [3] after clip, [4] after update,
@@ -268,6 +269,10 @@ int check_config(vd_ctx* ctx, const vd_config& c) {
     return ctx->fail(VD_ERR_ARG, "cull_depth must be in [0,256], got %d", c.cull_depth);
   if (c.site_tcap != 0 && (c.site_tcap < 8 || c.site_tcap > 32))
     return ctx->fail(VD_ERR_ARG, "site_tcap must be 0 or in [8,32], got %d", c.site_tcap);
+  if (c.cells_pcap != 0 && (c.cells_pcap < 8 || c.cells_pcap > c.pcap))
+    return ctx->fail(VD_ERR_ARG, "cells_pcap must be 0 or in [8,pcap], got %d", c.cells_pcap);
+  if (c.cells_tcap != 0 && (c.cells_tcap < 8 || c.cells_tcap > c.tcap))
+    return ctx->fail(VD_ERR_ARG, "cells_tcap must be 0 or in [8,tcap], got %d", c.cells_tcap);
   if (c.filter < 0 || c.filter > 2) return ctx->fail(VD_ERR_ARG, "filter must be 0, 1 or 2, got %d", c.filter);
   if (c.site_depth < 0 || c.site_depth > 256)
     return ctx->fail(VD_ERR_ARG, "site_depth must be in [0,256], got %d", c.site_depth);
@@ -369,6 +374,8 @@ void vd_config_default(vd_config* c) {
   c->record_pairs = 0;
   c->clip_blocks = 0;
   c->site_tcap = 0;
+  c->cells_pcap = 0;
+  c->cells_tcap = 0;
   c->cull_depth = 16;
   c->srnr_k = 0;
   c->filter = 0;
@@ -442,7 +449,7 @@ void vd_ctx_destroy(vd_ctx* ctx) {
   ctx->knn2.release();
   ctx->rec_meta.release(); ctx->rec_tris.release(); ctx->rec_count.release();
   ctx->c_status.release(); ctx->c_nv.release(); ctx->c_nt.release(); ctx->c_r2.release();
-  ctx->c_planes.release(); ctx->c_tris.release();
+  ctx->c_planes.release(); ctx->c_tris.release(); ctx->c_redo.release();
   for (auto& row : ctx->ev)
     for (auto& e : row)
       if (e) (void)hipEventDestroy(e);
@@ -792,10 +799,13 @@ int vd_clip_step(vd_ctx* ctx) {
       VD_HIP(ctx->c_r2.ensure(n), "alloc cells");
       VD_HIP(ctx->c_planes.ensure((size_t)n * c.pcap), "alloc cells");
       VD_HIP(ctx->c_tris.ensure((size_t)n * c.tcap), "alloc cells");
-      vd::CellStore cs{ctx->c_status.p, ctx->c_nv.p, ctx->c_nt.p, ctx->c_r2.p, ctx->c_planes.p, ctx->c_tris.p};
+      VD_HIP(ctx->c_redo.ensure((size_t)n + 1), "alloc cells");
+      vd::CellStore cs{ctx->c_status.p, ctx->c_nv.p, ctx->c_nt.p, ctx->c_r2.p, ctx->c_planes.p, ctx->c_tris.p,
+                       ctx->c_redo.p};
       a.cull_depth = c.cull_depth;
       if (c.cull_depth > 0) VD_HIP(ctx->p_surv.ensure(n), "alloc pairs");
-      VD_HIP(vd::cells_launch(a, cs, c.moments, c.accum, ctx->stream, ctx->p_ids.p, c.cull_depth > 0 ? ctx->p_surv.p : nullptr),
+      VD_HIP(vd::cells_launch(a, cs, c.moments, c.accum, ctx->stream, ctx->p_ids.p, c.cull_depth > 0 ? ctx->p_surv.p : nullptr,
+                              c.cells_pcap, c.cells_tcap),
              "cells");
       VD_HIP(record_phase(ctx, (5)), "event");
       VD_HIP(record_phase(ctx, (3)), "event");
@@ -825,8 +835,9 @@ int vd_clip_step(vd_ctx* ctx) {
     }
     a.tcap = fast_tcap;
     a.tcap_full = c.tcap;
-    a.cull_depth = c.cull_depth;
-    if (std::min(c.cull_depth, a.site_depth) > 0) {
+    // (a depth at which every skipped pair is what the configured caps give)
+    a.cull_depth = vd::exact_cull_depth(c.cull_depth, c.pcap, c.tcap);
+    if (std::min(a.cull_depth, a.site_depth) > 0) {
       // the emptiness test as a pass of its own; the clipping takes the survivors
       VD_HIP(ctx->p_surv.ensure(n), "alloc pairs");
       VD_HIP(vd::cull_sites(a, ctx->p_ids.p, ctx->p_surv.p, ctx->stream), "cull");

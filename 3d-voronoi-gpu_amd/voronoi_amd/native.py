@@ -43,6 +43,8 @@ class Config(ctypes.Structure):
         ("site_records", ctypes.c_int),
         ("site_depth", ctypes.c_int),
         ("knn_waves", ctypes.c_int),
+        ("cells_pcap", ctypes.c_int),
+        ("cells_tcap", ctypes.c_int),
     ]
 
 

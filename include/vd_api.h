@@ -76,8 +76,11 @@ typedef struct vd_config {
                          site_depth 40 */
   int cull_depth;     /* site-major form: a pair whose tet lies entirely beyond one of the
                          seed's first cull_depth bisectors (by more than the FP32 rounding
-                         bound plus 0.01) has an empty intersection and is not clipped;
-                         0 = off */
+                         bound plus 0.01) has an empty intersection and is not clipped
+                         (at most min(pcap - 4, (tcap - 3) / 2) bisectors, so that such a
+                         pair's clip could not have overflowed the caps first); cell form:
+                         the same test against the built cell, for cells whose clip by a
+                         tet could not overflow the caps; 0 = off */
   int srnr_k;         /* tet -> cell forms: 0 = the reference's behaviour (a pair whose cell
                          does not reach the security radius with site_k neighbours is
                          accumulated as it is, voronoi.cu:694-700); > site_k (at most 256):
@@ -101,6 +104,12 @@ typedef struct vd_config {
                          moved by an update need deeper tables) */
   int knn_waves;      /* wavefronts per workgroup of the kNN queries: 2, 4 or 8; 0 = automatic
                          (4); results do not depend on it */
+  int cells_pcap;     /* cell form: plane cap of kernel B's first launch (8..pcap); cells that
+                         fail there are built again at pcap/tcap, so results do not depend on
+                         it; 0 = automatic (40: with cells_tcap 64, two waves per CU instead
+                         of one at 64/96) */
+  int cells_tcap;     /* cell form: vertex cap of kernel B's first launch (8..tcap), likewise;
+                         0 = automatic (64) */
 } vd_config;
 
 typedef struct vd_timing {
@@ -119,7 +128,7 @@ typedef struct vd_ctx vd_ctx;
 /* Defaults: site_k 90, tet_k 0, pcap 64, tcap 96, VD_ACCUM_F32,
  * VD_MOMENTS_REFERENCE, VD_FORM_SITE_MAJOR, grid_density 3.1
  * (knearests.cu:243), record_pairs 0, clip_blocks 0, site_tcap 0, cull_depth 16, srnr_k 0,
- * filter 0, site_records 0, site_depth 0, knn_waves 0. */
+ * filter 0, site_records 0, site_depth 0, knn_waves 0, cells_pcap 0, cells_tcap 0. */
 void vd_config_default(vd_config* cfg);
 
 /* Context on HIP device `device`.  Replaces the device selection and buffer

@@ -285,6 +285,54 @@ def test_cells_form_bit_exact(vdn, O, case, accum, moments, cull):
         np.testing.assert_allclose(g["moments"], r["site_vals"], rtol=1e-5, atol=1e-3)
 
 
+@pytest.mark.parametrize("caps", [(12, 16, 64, 96), (24, 40, 64, 96), (12, 16, 20, 30)])
+def test_cells_kernel_b_redo(vdn, O, caps):
+    """Kernel B builds every cell at small caps first and the cells that fail
+    there again at the configured caps: with the first caps forced small (most
+    cells fail there; in the last case some fail at the configured caps too)
+    every pair equals the oracle's at the configured caps."""
+    fp, ft, pcap, tcap = caps
+    kind, n, ns = CASES[1]
+    verts, tets, sites = problem(kind, n, ns)
+    sk, tk = oracle_knn(kind, n, ns, 90, 64)
+    print(f"case={CASES[1]} caps={caps}", flush=True)
+    g = _run(vdn, verts, tets, sites, tet_k=64, form="cells", accum="f64", moments="robust", cull_depth=16,
+             pcap=pcap, tcap=tcap, cells_pcap=fp, cells_tcap=ft)
+    r = O.clip_cells(sites, sk, verts, tets, tk, geom="f32", pair_acc="f64", site_mode=O.SITE_FIXED,
+                     moments="robust", pcap=pcap, tcap=tcap)
+    bad = np.nonzero(g["status"].astype(np.int32) != r["pair_status"])[0]
+    assert bad.size == 0, f"{bad.size} pair statuses differ, first {bad[:5]}: {g['status'][bad[:5]]} vs {r['pair_status'][bad[:5]]}"
+    dv = np.nonzero(np.any(g["vals"] != r["pair_vals"], axis=1))[0]
+    assert dv.size == 0, f"{dv.size} pair values differ, first {dv[:5]}"
+    np.testing.assert_array_equal(g["moments"], r["site_vals"])
+    if caps[2:] == (20, 30):
+        assert np.any(r["cell_status"] != 4), "the configured caps should fail some cells here"
+
+
+@pytest.mark.parametrize("form", ["site", "cells"])
+def test_emptiness_test_at_small_caps(vdn, O, form):
+    """At caps where clipping overflows, a pair the emptiness test would skip
+    can overflow in the reference before it empties; the tests are limited so
+    that every pair still equals the oracle's at those caps."""
+    kind, n, ns = CASES[1]
+    verts, tets, sites = problem(kind, n, ns)
+    sk, tk = oracle_knn(kind, n, ns, 90, 64)
+    pcap, tcap = (12, 16) if form == "site" else (20, 30)
+    print(f"case={CASES[1]} form={form} caps={pcap}/{tcap}", flush=True)
+    g = _run(vdn, verts, tets, sites, tet_k=64, form=form, accum="f64", moments="robust", cull_depth=16,
+             pcap=pcap, tcap=tcap)
+    clip = O.clip_tets if form == "site" else O.clip_cells
+    r = clip(sites, sk, verts, tets, tk, geom="f32", pair_acc="f64", site_mode=O.SITE_FIXED, moments="robust",
+             pcap=pcap, tcap=tcap)
+    hist = np.bincount(r["pair_status"], minlength=7)
+    assert hist[0] + hist[1] > 0, "these caps should overflow some pairs"
+    bad = np.nonzero(g["status"].astype(np.int32) != r["pair_status"])[0]
+    assert bad.size == 0, f"{bad.size} pair statuses differ, first {bad[:5]}: {g['status'][bad[:5]]} vs {r['pair_status'][bad[:5]]}"
+    dv = np.nonzero(np.any(g["vals"] != r["pair_vals"], axis=1))[0]
+    assert dv.size == 0, f"{dv.size} pair values differ, first {dv[:5]}"
+    np.testing.assert_array_equal(g["moments"], r["site_vals"])
+
+
 def test_full_size_config3_torus(vdn, O):
     """820,272 torus tets / 200k Gaussian-mixture sites (50x density contrast) /
     TK=96: exact kNN, coverage away from security-radius failures, and an exact
