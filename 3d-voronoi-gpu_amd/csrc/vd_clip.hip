@@ -2044,8 +2044,15 @@ __global__ __launch_bounds__(64) void k_cells(ClipArgs a, CellStore cs, int pstr
   }
 }
 
+// tstride: the cell store's triangles per site; part 0 takes the sites whose
+// cell has fewer than split - 8 triangles (and the cells that were not
+// built), part 1 the other built cells.  Four cuts add at most 4 planes and 8
+// triangles to a cell (a closed polytope of this form has twice its planes
+// less 4 triangles), so part 0's pairs cannot reach a vertex cap of split and
+// part 0 runs at that smaller cap (less LDS per wave, more waves per CU) with
+// the results of the configured one.
 template <int MF, int ACC>
-__global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs) {
+__global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs, int tstride, int split, int part) {
   extern __shared__ __align__(16) uint8_t smem[];
   const int lane = threadIdx.x;
   // two cell slots, so that the pairs of consecutive sites share batches
@@ -2153,7 +2160,9 @@ __global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs) {
     const int cnv = cs.nv[s], cnt = cs.nt[s];
     const float cr2 = cs.r2[s];
     const P4 sp = site_at(a, s);
-    if (cst != kSuccess && cst != kSecurityRadiusNotReached) {
+    const bool built = cst == kSuccess || cst == kSecurityRadiusNotReached;
+    if ((part == 0) != (!built || cnt + 8 < split)) continue;  // the other launch's site
+    if (!built) {
       // the cell could not be built: every pair of the site takes its status
       for (unsigned i = pb + lane; i < pe; i += 64) {
         const unsigned pair = a.pair_ids[i];
@@ -2172,7 +2181,7 @@ __global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs) {
     if ((slot ? in_q1 : in_q0) > 0) clip_batch(queued);  // (queued < 64 here)
     __syncthreads();
     for (int i = lane; i < cnv; i += 64) cpl[slot * a.pcap + i] = cs.planes[(size_t)s * a.pcap + i];
-    for (int t = lane; t < cnt; t += 64) ctr[slot * a.tcap + t] = cs.tris[(size_t)s * a.tcap + t];
+    for (int t = lane; t < cnt; t += 64) ctr[slot * a.tcap + t] = cs.tris[(size_t)s * tstride + t];
     __syncthreads();
     if (slot) {
       site1 = s;
@@ -2592,6 +2601,11 @@ size_t cells_clip_lds(int pcap, int tcap) {
 #ifndef VD_CELLS_CLIP_BLOCKS
 #define VD_CELLS_CLIP_BLOCKS 8192
 #endif
+// Vertex cap of kernel C's first launch (the cells of fewer than 40 triangles).
+#ifndef VD_CELLS_CLIP_FAST_TCAP
+#define VD_CELLS_CLIP_FAST_TCAP 48
+#endif
+constexpr int kCellsClipFastTcap = VD_CELLS_CLIP_FAST_TCAP;
 constexpr int kCellsClipBlocks = VD_CELLS_CLIP_BLOCKS;
 
 hipError_t cells_launch(const ClipArgs& a0, const CellStore& cs, int moments, int acc_mode, hipStream_t st,
@@ -2631,11 +2645,23 @@ hipError_t cells_launch(const ClipArgs& a0, const CellStore& cs, int moments, in
     a.pair_count = cull_count;
   }
   lds = cells_clip_lds(a.pcap, a.tcap);
+  // the cells with fewer than kCellsClipFastTcap - 8 triangles (nearly all)
+  // at that vertex cap, then the others at the configured one
+  const int split = std::min(a.tcap, kCellsClipFastTcap);
   auto go = [&](auto k) -> hipError_t {
     hipError_t e2 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e2 != hipSuccess) return e2;
     // each wave takes every kCellsClipBlocks-th site
-    hipLaunchKernelGGL(k, dim3((unsigned)std::min(a.n_sites, kCellsClipBlocks)), dim3(64), lds, st, a, cs);
+    const dim3 grid((unsigned)std::min(a.n_sites, kCellsClipBlocks));
+    if (split < a.tcap) {
+      ClipArgs af = a;
+      af.tcap = split;
+      hipLaunchKernelGGL(k, grid, dim3(64), cells_clip_lds(af.pcap, af.tcap), st, af, cs, a.tcap, split, 0);
+      if ((e2 = hipGetLastError()) != hipSuccess) return e2;
+      hipLaunchKernelGGL(k, grid, dim3(64), lds, st, a, cs, a.tcap, split, 1);
+    } else {
+      hipLaunchKernelGGL(k, grid, dim3(64), lds, st, a, cs, a.tcap, INT_MAX, 0);
+    }
     return hipGetLastError();
   };
   if (moments == 0 && acc_mode == 0) return go(k_cells_clip<0, 0>);
