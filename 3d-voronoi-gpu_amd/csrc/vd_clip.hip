@@ -2601,9 +2601,13 @@ size_t cells_clip_lds(int pcap, int tcap) {
 #ifndef VD_CELLS_CLIP_BLOCKS
 #define VD_CELLS_CLIP_BLOCKS 8192
 #endif
-// Vertex cap of kernel C's first launch (the cells of fewer than 40 triangles).
+// Vertex cap of kernel C's first launch (the cells of fewer than 38
+// triangles): at 64 planes the largest whose LDS (23,040 bytes) lets 7 waves
+// share a CU (mean clipping ms of the cell form over iterations 1-3 at 1M
+// tets: 40 → 22.0, 44 → 19.8, 45 → 19.2, 46 → 19.1, 48 → 20.7; 27.6 in one
+// launch at 96).
 #ifndef VD_CELLS_CLIP_FAST_TCAP
-#define VD_CELLS_CLIP_FAST_TCAP 48
+#define VD_CELLS_CLIP_FAST_TCAP 46
 #endif
 constexpr int kCellsClipFastTcap = VD_CELLS_CLIP_FAST_TCAP;
 constexpr int kCellsClipBlocks = VD_CELLS_CLIP_BLOCKS;
@@ -2645,8 +2649,8 @@ hipError_t cells_launch(const ClipArgs& a0, const CellStore& cs, int moments, in
     a.pair_count = cull_count;
   }
   lds = cells_clip_lds(a.pcap, a.tcap);
-  // the cells with fewer than kCellsClipFastTcap - 8 triangles (nearly all)
-  // at that vertex cap, then the others at the configured one
+  // the cells with fewer than kCellsClipFastTcap - 8 triangles (most) at
+  // that vertex cap, then the others at the configured one
   const int split = std::min(a.tcap, kCellsClipFastTcap);
   auto go = [&](auto k) -> hipError_t {
     hipError_t e2 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
