@@ -154,19 +154,21 @@ struct vd_server {
 
   // Device bytes of a running job: its own buffers plus its share of the
   // batch's working memory (batch copies, the tets' start records, kNN
-  // tables, grouped pairs, retry list, grid, and the site-major form's
-  // volume-pass records: one per ten pairs).
+  // tables, grouped pairs, retry list, grid, the site-major form's
+  // volume-pass records: one per ten pairs, and the cell form's cells).
   size_t footprint(int nv, int nt, int ns) const {
     const size_t K = (size_t)cfg.site_k, TK = (size_t)cfg.tet_k;
     const size_t own = 16 * (size_t)nv + (16 + 48) * (size_t)nt + 16 * (size_t)ns + 4 * 256;
-    size_t rec = 0;
+    size_t rec = 0, cells = 0;
     if (cfg.form == VD_FORM_SITE_MAJOR && cfg.site_records >= 0) {
       const int ft = vdi::site_tcap_of(cfg);
       rec = (16 + 4 * (size_t)vd::rec_stride(ft)) * TK / 10;
     }
+    // status, counts, radius, planes, triangles and the rebuild list per site
+    if (cfg.form == VD_FORM_CELLS) cells = 20 + 16 * (size_t)cfg.pcap + 4 * (size_t)cfg.tcap;
     const size_t work =
         16 * (size_t)nv + (16 + 48 + 16 * vd::kTetPrepWords + 12 * TK + rec + 8) * (size_t)nt +
-        (16 + 4 * K + 64) * (size_t)ns;
+        (16 + 4 * K + 64 + cells) * (size_t)ns;
     return own + work;
   }
 
