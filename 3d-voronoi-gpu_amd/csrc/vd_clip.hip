@@ -2044,15 +2044,17 @@ __global__ __launch_bounds__(64) void k_cells(ClipArgs a, CellStore cs, int pstr
   }
 }
 
-// tstride: the cell store's triangles per site; part 0 takes the sites whose
-// cell has fewer than split - 8 triangles (and the cells that were not
-// built), part 1 the other built cells.  Four cuts add at most 4 planes and 8
-// triangles to a cell (a closed polytope of this form has twice its planes
-// less 4 triangles), so part 0's pairs cannot reach a vertex cap of split and
-// part 0 runs at that smaller cap (less LDS per wave, more waves per CU) with
-// the results of the configured one.
+// pstride / tstride: the cell store's planes and triangles per site; part 0
+// takes the sites whose cell has fewer than split - 8 triangles and at most
+// psplit - 4 planes (and the cells that were not built), part 1 the other
+// built cells.  Four cuts add at most 4 planes and 8 triangles to a cell (a
+// closed polytope of this form has twice its planes less 4 triangles), so
+// part 0's pairs cannot reach caps of psplit / split, and part 0 runs at
+// those smaller caps (less LDS per wave, more waves per CU) with the results
+// of the configured ones.
 template <int MF, int ACC>
-__global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs, int tstride, int split, int part) {
+__global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs, int pstride, int tstride, int split,
+                                                   int psplit, int part) {
   extern __shared__ __align__(16) uint8_t smem[];
   const int lane = threadIdx.x;
   // two cell slots, so that the pairs of consecutive sites share batches
@@ -2161,7 +2163,7 @@ __global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs, int
     const float cr2 = cs.r2[s];
     const P4 sp = site_at(a, s);
     const bool built = cst == kSuccess || cst == kSecurityRadiusNotReached;
-    if ((part == 0) != (!built || cnt + 8 < split)) continue;  // the other launch's site
+    if ((part == 0) != (!built || (cnt + 8 < split && cnv + 4 <= psplit))) continue;  // the other launch's site
     if (!built) {
       // the cell could not be built: every pair of the site takes its status
       for (unsigned i = pb + lane; i < pe; i += 64) {
@@ -2180,7 +2182,7 @@ __global__ __launch_bounds__(64) void k_cells_clip(ClipArgs a, CellStore cs, int
     slot ^= 1;
     if ((slot ? in_q1 : in_q0) > 0) clip_batch(queued);  // (queued < 64 here)
     __syncthreads();
-    for (int i = lane; i < cnv; i += 64) cpl[slot * a.pcap + i] = cs.planes[(size_t)s * a.pcap + i];
+    for (int i = lane; i < cnv; i += 64) cpl[slot * a.pcap + i] = cs.planes[(size_t)s * pstride + i];
     for (int t = lane; t < cnt; t += 64) ctr[slot * a.tcap + t] = cs.tris[(size_t)s * tstride + t];
     __syncthreads();
     if (slot) {
@@ -2601,15 +2603,20 @@ size_t cells_clip_lds(int pcap, int tcap) {
 #ifndef VD_CELLS_CLIP_BLOCKS
 #define VD_CELLS_CLIP_BLOCKS 8192
 #endif
-// Vertex cap of kernel C's first launch (the cells of fewer than 38
-// triangles): at 64 planes the largest whose LDS (23,040 bytes) lets 7 waves
-// share a CU (mean clipping ms of the cell form over iterations 1-3 at 1M
-// tets: 40 → 22.0, 44 → 19.8, 45 → 19.2, 46 → 19.1, 48 → 20.7; 27.6 in one
-// launch at 96).
+// Caps of kernel C's first launch, which takes the cells of fewer than 38
+// triangles and at most 32 planes: 36 planes and 46 vertices, 20,352 bytes of
+// LDS per wave, so that 8 waves share a CU (mean clipping ms of the cell form
+// over iterations 1-3 at 1M tets, the rest at 64/96: vertex cap 40 → 22.0,
+// 44 → 19.8, 45 → 19.2, 46 → 19.1, 48 → 20.7 at 64 planes (7 waves); 36/46 →
+// 17.8, 40/44 → 19.6, 40/46 → 19.5; 27.6 in one launch at 64/96).
 #ifndef VD_CELLS_CLIP_FAST_TCAP
 #define VD_CELLS_CLIP_FAST_TCAP 46
 #endif
 constexpr int kCellsClipFastTcap = VD_CELLS_CLIP_FAST_TCAP;
+#ifndef VD_CELLS_CLIP_FAST_PCAP
+#define VD_CELLS_CLIP_FAST_PCAP 36
+#endif
+constexpr int kCellsClipFastPcap = VD_CELLS_CLIP_FAST_PCAP;
 constexpr int kCellsClipBlocks = VD_CELLS_CLIP_BLOCKS;
 
 hipError_t cells_launch(const ClipArgs& a0, const CellStore& cs, int moments, int acc_mode, hipStream_t st,
@@ -2649,22 +2656,26 @@ hipError_t cells_launch(const ClipArgs& a0, const CellStore& cs, int moments, in
     a.pair_count = cull_count;
   }
   lds = cells_clip_lds(a.pcap, a.tcap);
-  // the cells with fewer than kCellsClipFastTcap - 8 triangles (most) at
-  // that vertex cap, then the others at the configured one
+  // the cells with fewer than kCellsClipFastTcap - 8 triangles and at most
+  // kCellsClipFastPcap - 4 planes (most) at those caps, then the others at
+  // the configured ones
   const int split = std::min(a.tcap, kCellsClipFastTcap);
+  const int psplit = std::min(a.pcap, kCellsClipFastPcap);
   auto go = [&](auto k) -> hipError_t {
     hipError_t e2 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e2 != hipSuccess) return e2;
     // each wave takes every kCellsClipBlocks-th site
     const dim3 grid((unsigned)std::min(a.n_sites, kCellsClipBlocks));
-    if (split < a.tcap) {
+    if (split < a.tcap || psplit < a.pcap) {
       ClipArgs af = a;
       af.tcap = split;
-      hipLaunchKernelGGL(k, grid, dim3(64), cells_clip_lds(af.pcap, af.tcap), st, af, cs, a.tcap, split, 0);
+      af.pcap = psplit;
+      hipLaunchKernelGGL(k, grid, dim3(64), cells_clip_lds(af.pcap, af.tcap), st, af, cs, a.pcap, a.tcap, split,
+                         psplit, 0);
       if ((e2 = hipGetLastError()) != hipSuccess) return e2;
-      hipLaunchKernelGGL(k, grid, dim3(64), lds, st, a, cs, a.tcap, split, 1);
+      hipLaunchKernelGGL(k, grid, dim3(64), lds, st, a, cs, a.pcap, a.tcap, split, psplit, 1);
     } else {
-      hipLaunchKernelGGL(k, grid, dim3(64), lds, st, a, cs, a.tcap, INT_MAX, 0);
+      hipLaunchKernelGGL(k, grid, dim3(64), lds, st, a, cs, a.pcap, a.tcap, INT_MAX, INT_MAX, 0);
     }
     return hipGetLastError();
   };
